@@ -1,0 +1,305 @@
+// Hand-written device templates for the MI355X (gfx950) stencil backend.
+//
+// Every JIT-built stencil module includes this header and supplies a traits
+// struct P (generated from the stencil IR) describing the fields, the
+// accesses and the per-cell expression.  The kernel template below does the
+// data movement; the expression is the only generated arithmetic.
+//
+// Execution model (one workgroup = NT threads = NT/64 wave64s):
+//   * Arrays are 3-D (s0, s1, s2), C-contiguous, interior (n0, n1, n2) at
+//     offset (h0, h1, h2).  1-D/2-D stencils are mapped to 3-D by leading
+//     unit dimensions, so axis 0 is the streaming axis and (1, 2) the plane.
+//   * A workgroup owns a TY x TZ tile of the plane and a chunk of `xch`
+//     planes along axis 0, and streams through the chunk plane by plane.
+//   * Accesses with no in-plane offset ("column" accesses, any dx) come from
+//     a per-thread register queue holding planes x+QLO .. x+QHI of the
+//     thread's own cells, so every plane is fetched from HBM once.
+//   * Accesses with an in-plane offset read an LDS tile of plane x+dx with a
+//     (RY, RZ) halo; the tile is loaded with row-contiguous 16-byte vector
+//     loads (RZ rounded up to the vector width so rows stay aligned).
+//   * The next plane's queue entries and LDS vectors are prefetched into
+//     registers before the current plane is computed, so HBM latency
+//     overlaps the arithmetic.
+//   * Workgroup ids are remapped so that each XCD (own L2) receives a
+//     contiguous range of tiles: neighbouring tiles share halos in one L2.
+//
+// Expression arithmetic is evaluated exactly as written (contraction off),
+// so results are bit-identical to NumPy on the same inputs.
+#pragma once
+#pragma clang fp contract(off)
+
+#define ST_HD __device__ __forceinline__
+#include "st_math.h"
+
+#ifndef ST_MAXF
+#define ST_MAXF 12
+#endif
+
+namespace st {
+
+typedef long long i64;
+typedef unsigned short bf16_t;
+
+// ---------------------------------------------------------------------------
+// storage <-> compute conversions
+// ---------------------------------------------------------------------------
+template <class S, class C> struct Cvt {
+  static ST_HD C ld(S v) { return (C)v; }
+  static ST_HD S st(C v) { return (S)v; }
+};
+template <> struct Cvt<bf16_t, float> {
+  static ST_HD float ld(bf16_t v) { return __builtin_bit_cast(float, ((unsigned)v) << 16); }
+  // round to nearest even; NaN kept quiet
+  static ST_HD bf16_t st(float f) {
+    unsigned u = __builtin_bit_cast(unsigned, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+  }
+};
+
+template <class T, int V> struct alignas(sizeof(T) * V) Vec { T v[V]; };
+
+template <class T, int V> ST_HD Vec<T, V> vload(const T* p) { return *reinterpret_cast<const Vec<T, V>*>(p); }
+template <class T, int V> ST_HD void vstore(T* p, const Vec<T, V>& x) { *reinterpret_cast<Vec<T, V>*>(p) = x; }
+
+// ---------------------------------------------------------------------------
+// launch arguments (one by-value struct; the host packs the same layout)
+// ---------------------------------------------------------------------------
+struct SweepArgs {
+  const void* in[ST_MAXF];
+  void* out[ST_MAXF];
+  i64 s0, s1, s2;   // array extents
+  int h0, h1, h2;   // array halo widths
+  int n0, n1, n2;   // interior extents
+  int ntz, nty, nch;  // tiles along z, y; chunks along x
+  int xch;          // planes per chunk
+  int nxcd;         // XCDs for the workgroup swizzle (1 = identity)
+};
+
+// Bijective remap of dispatch order -> logical tile so that XCD x (which
+// receives workgroups x, x+nxcd, ...) owns a contiguous logical range.
+ST_HD int xcd_swizzle(int i, int n, int nxcd) {
+  if (nxcd <= 1) return i;
+  int x = i % nxcd, j = i / nxcd;
+  int q = n / nxcd, r = n % nxcd;
+  return x * q + (x < r ? x : r) + j;
+}
+
+template <class P> struct Tab {
+  static constexpr int NTZ = P::TZ / P::VZ;
+  static constexpr int NTY = P::NT / NTZ;
+  static constexpr int KY = (P::TY + NTY - 1) / NTY;
+  static constexpr int QN = (P::QHI >= P::QLO) ? (P::QHI - P::QLO + 1) : 1;
+  static constexpr int rzv(int b) { return ((P::BRZ[b] + P::VZ - 1) / P::VZ) * P::VZ; }
+  static constexpr int rows(int b) { return P::TY + 2 * P::BRY[b]; }
+  static constexpr int cols(int b) { return P::TZ + 2 * rzv(b); }
+  static constexpr int vecs(int b) { return rows(b) * cols(b) / P::VZ; }
+  static constexpr int nvb(int b) { return (vecs(b) + P::NT - 1) / P::NT; }
+  static constexpr int off(int b) { int o = 0; for (int k = 0; k < b; ++k) o += rows(k) * cols(k); return o; }
+  static constexpr int lds_elems() { int o = 0; for (int k = 0; k < P::NB; ++k) o += rows(k) * cols(k); return o > 0 ? o : 1; }
+  static constexpr int max_nvb() { int m = 1; for (int k = 0; k < P::NB; ++k) m = nvb(k) > m ? nvb(k) : m; return m; }
+  static constexpr int buf(int f, int dx) {
+    for (int k = 0; k < P::NB; ++k) if (P::BF[k] == f && P::BDX[k] == dx) return k;
+    return -1;
+  }
+};
+
+// Per-cell view handed to the generated expression.
+template <class P> struct Cell {
+  using C = typename P::C;
+  using T = Tab<P>;
+  const C (*q)[T::QN][T::KY][P::VZ];
+  const C* lds;
+  int k, v, ly, lz;
+
+  template <int F, int DX, int DY, int DZ> ST_HD C g() const {
+    if constexpr (DY == 0 && DZ == 0 && P::QHI >= P::QLO && DX >= P::QLO && DX <= P::QHI) {
+      return q[F][DX - P::QLO][k][v];
+    } else {
+      constexpr int b = T::buf(F, DX);
+      static_assert(b >= 0, "access has no LDS buffer");
+      return lds[T::off(b) + (ly + P::BRY[b] + DY) * T::cols(b) + (lz + T::rzv(b) + DZ)];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// the plane-streaming sweep
+// ---------------------------------------------------------------------------
+template <class P>
+ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
+  using S = typename P::S;
+  using C = typename P::C;
+  using T = Tab<P>;
+  constexpr int NT = P::NT, TY = P::TY, TZ = P::TZ, VZ = P::VZ, NF = P::NF, NO = P::NO, NB = P::NB;
+  constexpr int NTZ = T::NTZ, NTY = T::NTY, KY = T::KY, QN = T::QN, NVB = T::max_nvb();
+  static_assert(P::NT % NTZ == 0, "NT must be a multiple of TZ/VZ");
+
+  __shared__ C lds[T::lds_elems()];
+
+  const int tid = threadIdx.x;
+  const int tz = tid % NTZ, ty = tid / NTZ;
+  const int z0 = tile_z * TZ, y0 = tile_y * TY;
+  const int xb = chunk * a.xch;
+  const int xe = (xb + a.xch < a.n0) ? xb + a.xch : a.n0;
+  const i64 s1 = a.s1, s2 = a.s2;
+  const int zc = z0 + tz * VZ;                 // first interior z of my vector
+  const bool zin = zc < a.n2;
+
+  C q[NF][QN][KY][VZ];
+  C pq[NF][KY][VZ];
+  C pb[NB > 0 ? NB : 1][NVB][VZ];
+
+  auto plane_ptr = [&](int f, int x) -> const S* {
+    return reinterpret_cast<const S*>(a.in[f]) + (i64)(a.h0 + x) * s1 * s2;
+  };
+
+  // load my cells of plane x of field f into dst
+  auto load_cells = [&](int f, int x, C (&dst)[KY][VZ]) {
+    const S* p = plane_ptr(f, x);
+#pragma unroll
+    for (int k = 0; k < KY; ++k) {
+      const int y = y0 + ty + k * NTY;
+      if (zin && y < a.n1 && ty + k * NTY < TY) {
+        Vec<S, VZ> w = vload<S, VZ>(p + (i64)(a.h1 + y) * s2 + a.h2 + zc);
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
+      } else {
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) dst[k][v] = C(0);
+      }
+    }
+  };
+
+  // prefetch the LDS vectors of plane x (tile + halo) for every buffer
+  auto load_bufs = [&](int x) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const S* p = plane_ptr(P::BF[b], x + P::BDX[b]);
+      const int vpr = T::cols(b) / VZ;
+#pragma unroll
+      for (int j = 0; j < NVB; ++j) {
+        const int i = tid + j * NT;
+        bool ok = (j < T::nvb(b)) && (i < T::vecs(b));
+        const int r = i / vpr, c = i % vpr;
+        const i64 gy = (i64)a.h1 + y0 - P::BRY[b] + r;
+        const i64 gz = (i64)a.h2 + z0 - T::rzv(b) + c * VZ;
+        ok = ok && gy >= 0 && gy < s1 && gz >= 0 && gz + VZ <= s2;
+        if (ok) {
+          Vec<S, VZ> w = vload<S, VZ>(p + gy * s2 + gz);
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) pb[b][j][v] = Cvt<S, C>::ld(w.v[v]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) pb[b][j][v] = C(0);
+        }
+      }
+    }
+  };
+
+  auto store_bufs = [&]() {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+#pragma unroll
+      for (int j = 0; j < NVB; ++j) {
+        const int i = tid + j * NT;
+        if (j < T::nvb(b) && i < T::vecs(b)) {
+          C* d = lds + T::off(b) + i * VZ;
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) d[v] = pb[b][j][v];
+        }
+      }
+    }
+  };
+
+  // ---- prologue: queue planes xb+QLO .. xb+QHI-1, prefetch plane xb+QHI
+  if constexpr (P::QHI >= P::QLO) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (!P::QF[f]) continue;
+#pragma unroll
+      for (int i = 0; i + 1 < QN; ++i) load_cells(f, xb + P::QLO + i, q[f][i + 1]);
+      load_cells(f, xb + P::QHI, pq[f]);
+    }
+  }
+  load_bufs(xb);
+
+  for (int x = xb; x < xe; ++x) {
+    // install plane x+QHI at the back of the queue
+    if constexpr (P::QHI >= P::QLO) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        if (!P::QF[f]) continue;
+#pragma unroll
+        for (int i = 0; i + 1 < QN; ++i)
+#pragma unroll
+          for (int k = 0; k < KY; ++k)
+#pragma unroll
+            for (int v = 0; v < VZ; ++v) q[f][i][k][v] = q[f][i + 1][k][v];
+#pragma unroll
+        for (int k = 0; k < KY; ++k)
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) q[f][QN - 1][k][v] = pq[f][k][v];
+      }
+    }
+    if constexpr (NB > 0) {
+      __syncthreads();
+      store_bufs();
+      __syncthreads();
+    }
+    // prefetch plane x+1 while plane x is computed
+    if (x + 1 < xe) {
+      if constexpr (P::QHI >= P::QLO) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+          if (P::QF[f]) load_cells(f, x + 1 + P::QHI, pq[f]);
+      }
+      load_bufs(x + 1);
+    }
+
+#pragma unroll
+    for (int k = 0; k < KY; ++k) {
+      const int lyk = ty + k * NTY;
+      const int y = y0 + lyk;
+      if (lyk >= TY || y >= a.n1 || !zin) continue;
+      C res[NO][VZ];
+#pragma unroll
+      for (int v = 0; v < VZ; ++v) {
+        Cell<P> c{q, lds, k, v, lyk, tz * VZ + v};
+        C o[NO];
+        P::eval(c, o);
+#pragma unroll
+        for (int m = 0; m < NO; ++m) res[m][v] = o[m];
+      }
+      const i64 off = ((i64)(a.h0 + x) * s1 + (a.h1 + y)) * s2 + a.h2 + zc;
+#pragma unroll
+      for (int m = 0; m < NO; ++m) {
+        S* dst = reinterpret_cast<S*>(a.out[m]) + off;
+        if (zc + VZ <= a.n2) {
+          Vec<S, VZ> w;
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) w.v[v] = Cvt<S, C>::st(res[m][v]);
+          vstore<S, VZ>(dst, w);
+        } else {
+#pragma unroll
+          for (int v = 0; v < VZ; ++v)
+            if (zc + v < a.n2) dst[v] = Cvt<S, C>::st(res[m][v]);
+        }
+      }
+    }
+  }
+}
+
+template <class P>
+ST_HD void sweep_entry(const SweepArgs& a) {
+  const int n = a.ntz * a.nty * a.nch;
+  const int t = xcd_swizzle((int)blockIdx.x, n, a.nxcd);
+  const int tile_z = t % a.ntz;
+  const int rest = t / a.ntz;
+  const int tile_y = rest % a.nty;
+  const int chunk = rest / a.nty;
+  sweep_tile<P>(a, tile_z, tile_y, chunk);
+}
+
+}  // namespace st

@@ -1,0 +1,562 @@
+// libstencil_rt.so -- C-ABI runtime of the MI355X stencil backend.
+//
+// Host side: hiprtc compilation with an in-process and on-disk code-object
+// cache, module/function handles, by-value-argument launches, events,
+// hipGraph capture, device queries and kernel resource queries.
+// Device side (compiled ahead of time for gfx950): halo fill, vector copy,
+// interior max-norm differences and flat<->brick layout conversion.
+// See include/stencil_rt.h for the contract of every entry point.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <unistd.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/stencil_rt.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                             \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return fail((int)e_, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+  } while (0)
+
+uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull) {
+  const unsigned char* p = static_cast<const unsigned char*>(data);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= p[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+std::mutex g_mu;
+std::map<std::string, hipModule_t> g_modules;  // key: device + hash
+
+int do_compile(const char* src, const char* name, const char* const* opts, int nopts,
+               std::vector<char>& code) {
+  hiprtcProgram prog;
+  hiprtcResult r = hiprtcCreateProgram(&prog, src, name ? name : "stencil.hip", 0, nullptr, nullptr);
+  if (r != HIPRTC_SUCCESS) return fail(ST_E_COMPILE, "hiprtcCreateProgram: %s", hiprtcGetErrorString(r));
+  r = hiprtcCompileProgram(prog, nopts, const_cast<const char**>(opts));
+  size_t log_size = 0;
+  hiprtcGetProgramLogSize(prog, &log_size);
+  std::string log(log_size + 1, '\0');
+  if (log_size) hiprtcGetProgramLog(prog, &log[0]);
+  if (r != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    g_err = std::string("hiprtc compile failed: ") + hiprtcGetErrorString(r) + "\n" + log;
+    return ST_E_COMPILE;
+  }
+  size_t size = 0;
+  hiprtcGetCodeSize(prog, &size);
+  code.resize(size);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  g_err = log;
+  return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fixed device kernels
+// ---------------------------------------------------------------------------
+
+template <class E>
+__global__ void k_fill_side(E* u, int64_t outer, int64_t n, int64_t inner, int64_t h, int kind,
+                            E value, int side_mask) {
+  // element e of the (side, k, o, i) slab space
+  const int64_t per_side = outer * h * inner;
+  const int64_t total = per_side * 2;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int side = (int)(e / per_side);
+    if (!((side_mask >> side) & 1)) continue;
+    int64_t r = e % per_side;
+    const int64_t o = r / (h * inner);
+    r %= h * inner;
+    const int64_t k = r / inner;
+    const int64_t i = r % inner;
+    int64_t dst, src = -1;
+    if (side == 0) {
+      dst = k;
+      if (kind == ST_BC_NEUMANN) src = h;
+      else if (kind == ST_BC_PERIODIC) src = n - 2 * h + k;
+      else if (kind == ST_BC_REFLECTING) src = 2 * h - 1 - k;
+    } else {
+      dst = n - h + k;
+      if (kind == ST_BC_NEUMANN) src = n - h - 1;
+      else if (kind == ST_BC_PERIODIC) src = h + k;
+      else if (kind == ST_BC_REFLECTING) src = n - h - 1 - k;
+    }
+    E* base = u + o * n * inner + i;
+    base[dst * inner] = (kind == ST_BC_DIRICHLET) ? value : base[src * inner];
+  }
+}
+
+__global__ void k_copy16(uint4* dst, const uint4* src, size_t n16) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+__global__ void k_copy1(unsigned char* dst, const unsigned char* src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__device__ inline double load_as_f64(const void* p, int dtype, int64_t i) {
+  if (dtype == ST_F64) return static_cast<const double*>(p)[i];
+  if (dtype == ST_F32) return (double)static_cast<const float*>(p)[i];
+  unsigned short b = static_cast<const unsigned short*>(p)[i];
+  return (double)__builtin_bit_cast(float, ((unsigned)b) << 16);
+}
+
+struct Shape4 {
+  int64_t s[4];
+  int64_t h[4];
+  int64_t n[4];
+};
+
+// out[0] = max |a-b| (as ordered uint64 bits), out[1] = max |b|, out[2] = NaN seen
+__global__ void k_diff_norms(const void* a, int da, const void* b, int db, Shape4 sh,
+                             unsigned long long* out) {
+  const int64_t total = sh.n[0] * sh.n[1] * sh.n[2] * sh.n[3];
+  double md = 0.0, mb = 0.0;
+  unsigned nan = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int64_t r = e, idx = 0;
+    int64_t c[4];
+    for (int d = 3; d >= 0; --d) {
+      c[d] = r % sh.n[d];
+      r /= sh.n[d];
+    }
+    for (int d = 0; d < 4; ++d) idx = idx * sh.s[d] + (c[d] + sh.h[d]);
+    const double x = load_as_f64(a, da, idx), y = load_as_f64(b, db, idx);
+    const double df = fabs(x - y), ab = fabs(y);
+    if (df != df || ab != ab) nan = 1;
+    md = df > md ? df : md;
+    mb = ab > mb ? ab : mb;
+  }
+  // wave reduction then one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) {
+    const double od = __shfl_down(md, off), ob = __shfl_down(mb, off);
+    const unsigned on = __shfl_down(nan, off);
+    md = od > md ? od : md;
+    mb = ob > mb ? ob : mb;
+    nan |= on;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    const unsigned long long bd = __builtin_bit_cast(unsigned long long, md);
+    const unsigned long long bb = __builtin_bit_cast(unsigned long long, mb);
+    atomicMax(&out[0], bd);
+    atomicMax(&out[1], bb);
+    if (nan) atomicOr(&out[2], 1ull);
+  }
+}
+
+template <class E>
+__global__ void k_bricks(E* flat, E* bricked, int ndim, Shape4 sh, int64_t nb0, int64_t nb1,
+                         int64_t nb2, int bs, int to_bricks, int64_t total) {
+  const int64_t nb[3] = {nb0, nb1, nb2};
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    // e indexes the bricked array: (b_0..b_{d-1}, o_0..o_{d-1})
+    int64_t r = e;
+    int64_t o[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+    for (int d = ndim - 1; d >= 0; --d) {
+      o[d] = r % bs;
+      r /= bs;
+    }
+    for (int d = ndim - 1; d >= 0; --d) {
+      b[d] = r % nb[d];
+      r /= nb[d];
+    }
+    bool in = true;
+    int64_t fi = 0;
+    for (int d = 0; d < ndim; ++d) {
+      const int64_t g = b[d] * bs + o[d];
+      in = in && g < sh.s[d];
+      fi = fi * sh.s[d] + g;
+    }
+    if (to_bricks) bricked[e] = in ? flat[fi] : E(0);
+    else if (in) flat[fi] = bricked[e];
+  }
+}
+
+unsigned grid_for(int64_t n, int bs = 256) {
+  int64_t g = (n + bs - 1) / bs;
+  if (g < 1) g = 1;
+  if (g > 65536) g = 65536;
+  return (unsigned)g;
+}
+
+size_t esize(int dtype) { return dtype == ST_F64 ? 8 : dtype == ST_F32 ? 4 : 2; }
+
+}  // namespace
+
+extern "C" {
+
+const char* st_last_error(void) { return g_err.c_str(); }
+
+int st_init(int dev) {
+  HIPCHK(hipSetDevice(dev));
+  return ST_OK;
+}
+
+int st_device_count(int* count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return ST_OK;
+}
+
+int st_device_info_get(int dev, st_device_info* out) {
+  if (!out) return fail(ST_E_INVALID, "null output");
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, dev));
+  memset(out, 0, sizeof(*out));
+  snprintf(out->name, sizeof(out->name), "%s", p.name);
+  snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+  out->compute_units = p.multiProcessorCount;
+  out->max_threads_per_block = p.maxThreadsPerBlock;
+  out->lds_per_block = p.sharedMemPerBlock;
+  out->total_mem = p.totalGlobalMem;
+  out->clock_khz = p.clockRate;
+  out->mem_clock_khz = p.memoryClockRate;
+  out->mem_bus_width = p.memoryBusWidth;
+  out->warp_size = p.warpSize;
+  int cur = 0;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur == dev) {
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) == hipSuccess) out->free_mem = f;
+  }
+  return ST_OK;
+}
+
+int st_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
+  return ST_OK;
+}
+
+int st_compile_only(const char* src, const char* name, const char* const* opts, int nopts,
+                    void** code, size_t* size) {
+  if (!src || !code || !size) return fail(ST_E_INVALID, "null argument");
+  std::vector<char> buf;
+  int rc = do_compile(src, name, opts, nopts, buf);
+  if (rc) return rc;
+  void* p = malloc(buf.size());
+  if (!p) return fail(ST_E_INVALID, "out of host memory");
+  memcpy(p, buf.data(), buf.size());
+  *code = p;
+  *size = buf.size();
+  return ST_OK;
+}
+
+void st_free(void* p) { free(p); }
+
+int st_load_code(const void* code, size_t size, void** module) {
+  (void)size;
+  hipModule_t m;
+  HIPCHK(hipModuleLoadData(&m, code));
+  *module = (void*)m;
+  return ST_OK;
+}
+
+int st_compile(const char* src, const char* name, const char* const* opts, int nopts,
+               const char* cache_dir, void** module) {
+  if (!src || !module) return fail(ST_E_INVALID, "null argument");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  uint64_t h = fnv1a(src, strlen(src));
+  for (int i = 0; i < nopts; ++i) {
+    h = fnv1a(opts[i], strlen(opts[i]), h);
+    h = fnv1a("\x1f", 1, h);
+  }
+  char key[64];
+  snprintf(key, sizeof(key), "%016llx", (unsigned long long)h);
+  const std::string mkey = std::to_string(dev) + ":" + key;
+  std::lock_guard<std::mutex> lock(g_mu);
+  auto it = g_modules.find(mkey);
+  if (it != g_modules.end()) {
+    *module = (void*)it->second;
+    return ST_OK;
+  }
+  std::vector<char> code;
+  std::string path;
+  if (cache_dir && *cache_dir) {
+    path = std::string(cache_dir) + "/" + key + ".hsaco";
+    FILE* f = fopen(path.c_str(), "rb");
+    if (f) {
+      fseek(f, 0, SEEK_END);
+      long n = ftell(f);
+      fseek(f, 0, SEEK_SET);
+      if (n > 0) {
+        code.resize((size_t)n);
+        if (fread(code.data(), 1, (size_t)n, f) != (size_t)n) code.clear();
+      }
+      fclose(f);
+    }
+  }
+  if (code.empty()) {
+    int rc = do_compile(src, name, opts, nopts, code);
+    if (rc) return rc;
+    if (!path.empty()) {
+      std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+      FILE* f = fopen(tmp.c_str(), "wb");
+      if (f) {
+        size_t w = fwrite(code.data(), 1, code.size(), f);
+        fclose(f);
+        if (w == code.size()) rename(tmp.c_str(), path.c_str());
+        else remove(tmp.c_str());
+      }
+    }
+  }
+  hipModule_t m;
+  HIPCHK(hipModuleLoadData(&m, code.data()));
+  g_modules[mkey] = m;
+  *module = (void*)m;
+  return ST_OK;
+}
+
+int st_unload(void* module) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  for (auto it = g_modules.begin(); it != g_modules.end(); ++it)
+    if ((void*)it->second == module) {
+      g_modules.erase(it);
+      break;
+    }
+  HIPCHK(hipModuleUnload((hipModule_t)module));
+  return ST_OK;
+}
+
+int st_get_function(void* module, const char* kernel, void** function) {
+  hipFunction_t f;
+  hipError_t e = hipModuleGetFunction(&f, (hipModule_t)module, kernel);
+  if (e != hipSuccess) return fail(ST_E_NOTFOUND, "kernel '%s' not found: %s", kernel, hipGetErrorString(e));
+  *function = (void*)f;
+  return ST_OK;
+}
+
+int st_launch(void* function, unsigned gx, unsigned gy, unsigned gz, unsigned bx, unsigned by,
+              unsigned bz, unsigned dyn_lds, void* stream, const void* args, size_t args_size) {
+  if (!function) return fail(ST_E_INVALID, "null function");
+  if (gx == 0 || gy == 0 || gz == 0) return ST_OK;  // empty launch: nothing to do
+  size_t sz = args_size;
+  void* config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, const_cast<void*>(args),
+                    HIP_LAUNCH_PARAM_BUFFER_SIZE, &sz, HIP_LAUNCH_PARAM_END};
+  HIPCHK(hipModuleLaunchKernel((hipFunction_t)function, gx, gy, gz, bx, by, bz, dyn_lds,
+                               (hipStream_t)stream, nullptr, config));
+  return ST_OK;
+}
+
+int st_function_resources(void* function, st_resources* out) {
+  if (!function || !out) return fail(ST_E_INVALID, "null argument");
+  hipFunction_t f = (hipFunction_t)function;
+  memset(out, 0, sizeof(*out));
+  HIPCHK(hipFuncGetAttribute(&out->num_regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, f));
+  HIPCHK(hipFuncGetAttribute(&out->local_bytes, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, f));
+  HIPCHK(hipFuncGetAttribute(&out->shared_bytes, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, f));
+  HIPCHK(hipFuncGetAttribute(&out->max_threads, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, f));
+  HIPCHK(hipFuncGetAttribute(&out->const_bytes, HIP_FUNC_ATTRIBUTE_CONST_SIZE_BYTES, f));
+  return ST_OK;
+}
+
+int st_fill_halo(void* ptr, int dtype, int ndim, const int64_t* shape, const int64_t* halo,
+                 const int* kinds, const double* values, void* stream) {
+  if (!ptr || ndim < 1 || ndim > 4) return fail(ST_E_INVALID, "bad fill_halo arguments");
+  hipStream_t s = (hipStream_t)stream;
+  for (int d = 0; d < ndim; ++d) {
+    const int64_t h = halo[d], n = shape[d];
+    if (h == 0) continue;
+    if (n < 2 * h) return fail(ST_E_INVALID, "dim %d: extent %lld < 2*halo", d, (long long)n);
+    int64_t outer = 1, inner = 1;
+    for (int k = 0; k < d; ++k) outer *= shape[k];
+    for (int k = d + 1; k < ndim; ++k) inner *= shape[k];
+    const int lk = kinds[2 * d], hk = kinds[2 * d + 1];
+    // Sides of one dim read/write disjoint layers when the interior is at
+    // least h wide; then one launch per kind does both, else low then high.
+    const bool split = (n - 2 * h) < h || lk != hk || values[2 * d] != values[2 * d + 1];
+    for (int pass = 0; pass < (split ? 2 : 1); ++pass) {
+      const int kind = split ? (pass == 0 ? lk : hk) : lk;
+      const int mask = split ? (1 << pass) : 3;
+      const double val = split ? values[2 * d + pass] : values[2 * d];
+      if (kind == ST_BC_NONE) continue;
+      const unsigned g = grid_for(2 * outer * h * inner);
+      if (dtype == ST_F64) {
+        hipLaunchKernelGGL(k_fill_side<double>, dim3(g), dim3(256), 0, s, (double*)ptr, outer, n,
+                           inner, h, kind, val, mask);
+      } else if (dtype == ST_F32) {
+        hipLaunchKernelGGL(k_fill_side<float>, dim3(g), dim3(256), 0, s, (float*)ptr, outer, n,
+                           inner, h, kind, (float)val, mask);
+      } else {
+        float fv = (float)val;
+        unsigned u;
+        memcpy(&u, &fv, 4);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        hipLaunchKernelGGL(k_fill_side<unsigned short>, dim3(g), dim3(256), 0, s,
+                           (unsigned short*)ptr, outer, n, inner, h, kind,
+                           (unsigned short)(u >> 16), mask);
+      }
+      HIPCHK(hipGetLastError());
+    }
+  }
+  return ST_OK;
+}
+
+int st_copy(void* dst, const void* src, size_t nbytes, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  const bool aligned = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0;
+  size_t done = 0;
+  if (aligned && nbytes >= 16) {
+    const size_t n16 = nbytes / 16;
+    hipLaunchKernelGGL(k_copy16, dim3(grid_for((int64_t)n16, 256)), dim3(256), 0, s, (uint4*)dst,
+                       (const uint4*)src, n16);
+    done = n16 * 16;
+  }
+  if (done < nbytes) {
+    hipLaunchKernelGGL(k_copy1, dim3(grid_for((int64_t)(nbytes - done))), dim3(256), 0, s,
+                       (unsigned char*)dst + done, (const unsigned char*)src + done, nbytes - done);
+  }
+  HIPCHK(hipGetLastError());
+  return ST_OK;
+}
+
+int st_diff_norms(const void* a, int dtype_a, const void* b, int dtype_b, int ndim,
+                  const int64_t* shape, const int64_t* halo, void* stream, double* out) {
+  if (ndim < 1 || ndim > 4 || !out) return fail(ST_E_INVALID, "bad diff_norms arguments");
+  Shape4 sh;
+  for (int d = 0; d < 4; ++d) {
+    const int k = d - (4 - ndim);
+    sh.s[d] = k >= 0 ? shape[k] : 1;
+    sh.h[d] = k >= 0 ? halo[k] : 0;
+    sh.n[d] = sh.s[d] - 2 * sh.h[d];
+    if (sh.n[d] < 0) return fail(ST_E_INVALID, "negative interior");
+  }
+  hipStream_t s = (hipStream_t)stream;
+  static thread_local void* buf = nullptr;
+  static thread_local int buf_dev = -1;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (!buf || buf_dev != dev) {
+    HIPCHK(hipMalloc(&buf, 3 * sizeof(unsigned long long)));
+    buf_dev = dev;
+  }
+  HIPCHK(hipMemsetAsync(buf, 0, 3 * sizeof(unsigned long long), s));
+  const int64_t total = sh.n[0] * sh.n[1] * sh.n[2] * sh.n[3];
+  unsigned g = grid_for(total);
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(k_diff_norms, dim3(g), dim3(256), 0, s, a, dtype_a, b, dtype_b, sh,
+                     (unsigned long long*)buf);
+  HIPCHK(hipGetLastError());
+  unsigned long long host[3];
+  HIPCHK(hipMemcpyAsync(host, buf, sizeof(host), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  memcpy(&out[0], &host[0], 8);
+  memcpy(&out[1], &host[1], 8);
+  if (host[2]) out[0] = out[1] = __builtin_nan("");
+  return ST_OK;
+}
+
+int st_bricks(void* flat, void* bricked, int dtype, int ndim, const int64_t* shape, int bs,
+              int to_bricks, void* stream) {
+  if (ndim < 1 || ndim > 3 || bs < 1) return fail(ST_E_INVALID, "bad bricks arguments");
+  Shape4 sh;
+  int64_t nb[3] = {1, 1, 1};
+  int64_t total = 1;
+  for (int d = 0; d < ndim; ++d) {
+    sh.s[d] = shape[d];
+    nb[d] = (shape[d] + bs - 1) / bs;
+    total *= nb[d] * bs;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned g = grid_for(total);
+  if (esize(dtype) == 8)
+    hipLaunchKernelGGL(k_bricks<double>, dim3(g), dim3(256), 0, s, (double*)flat, (double*)bricked,
+                       ndim, sh, nb[0], nb[1], nb[2], bs, to_bricks, total);
+  else if (esize(dtype) == 4)
+    hipLaunchKernelGGL(k_bricks<float>, dim3(g), dim3(256), 0, s, (float*)flat, (float*)bricked, ndim,
+                       sh, nb[0], nb[1], nb[2], bs, to_bricks, total);
+  else
+    hipLaunchKernelGGL(k_bricks<unsigned short>, dim3(g), dim3(256), 0, s, (unsigned short*)flat,
+                       (unsigned short*)bricked, ndim, sh, nb[0], nb[1], nb[2], bs, to_bricks, total);
+  HIPCHK(hipGetLastError());
+  return ST_OK;
+}
+
+int st_event_create(void** ev) {
+  hipEvent_t e;
+  HIPCHK(hipEventCreate(&e));
+  *ev = (void*)e;
+  return ST_OK;
+}
+int st_event_record(void* ev, void* stream) {
+  HIPCHK(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream));
+  return ST_OK;
+}
+int st_event_elapsed_ms(void* start, void* stop, float* ms) {
+  HIPCHK(hipEventSynchronize((hipEvent_t)stop));
+  HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return ST_OK;
+}
+int st_event_destroy(void* ev) {
+  HIPCHK(hipEventDestroy((hipEvent_t)ev));
+  return ST_OK;
+}
+int st_stream_sync(void* stream) {
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  return ST_OK;
+}
+int st_device_sync(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return ST_OK;
+}
+
+int st_graph_begin(void* stream) {
+  HIPCHK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  return ST_OK;
+}
+int st_graph_end(void* stream, void** graph_exec) {
+  hipGraph_t g;
+  HIPCHK(hipStreamEndCapture((hipStream_t)stream, &g));
+  hipGraphExec_t ex;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) return fail((int)e, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  *graph_exec = (void*)ex;
+  return ST_OK;
+}
+int st_graph_launch(void* graph_exec, void* stream) {
+  HIPCHK(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+  return ST_OK;
+}
+int st_graph_destroy(void* graph_exec) {
+  HIPCHK(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+  return ST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+"""HIP emitter: stencil metadata -> HIP C++ for gfx950 (replaces the cuTile
+emitter ``cutile/lowering/target_to_python.py:34-526``).
+
+The emitted translation unit includes the hand-written device templates in
+``csrc/stencil_core.h`` and defines, per variant, a traits struct ``P_<v>``
+(fields, access tables, tile shape, storage/compute types and the per-cell
+expression) plus an ``extern "C"`` kernel that instantiates the sweep
+template.  Variants differ in the contiguous-axis vector width (16-byte
+vectors when the array rows allow it, scalars otherwise) and, for temporal
+blocking, in the depth.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Sequence
+
+from cutile.lowering.meta import StencilMeta, access_table
+
+STORAGE_CTYPES = {"float64": "double", "float32": "float", "bfloat16": "st::bf16_t"}
+COMPUTE_CTYPES = {"float64": "double", "float32": "float"}
+ELEM_BYTES = {"float64": 8, "float32": 4, "bfloat16": 2}
+
+
+@dataclass(frozen=True)
+class KernelConfig:
+    """Launch geometry of one sweep variant.
+
+    ``ty``/``tz`` are the in-plane tile (rows x contiguous columns) of a
+    workgroup, ``vz`` the contiguous-axis vector width, ``nt`` the threads per
+    workgroup (a multiple of 64)."""
+
+    ty: int
+    tz: int
+    vz: int
+    nt: int
+
+    @property
+    def tag(self) -> str:
+        return f"t{self.ty}x{self.tz}v{self.vz}n{self.nt}"
+
+
+def plan_accesses(meta: StencilMeta) -> dict:
+    """Split accesses into register-queue columns and LDS plane buffers."""
+    tab = access_table(meta)
+    nf = meta.num_inputs
+    col = [t for t in tab if t[2] == 0 and t[3] == 0]
+    qlo = min((t[1] for t in col), default=1)
+    qhi = max((t[1] for t in col), default=0)
+    qf = [0] * nf
+    for f, dx, dy, dz in col:
+        qf[f] = 1
+    bufs: dict = {}
+    for f, dx, dy, dz in tab:
+        if dy == 0 and dz == 0:
+            continue
+        ry, rz = bufs.get((f, dx), (0, 0))
+        bufs[(f, dx)] = (max(ry, abs(dy)), max(rz, abs(dz)))
+    blist = sorted(bufs.items())
+    return {"qlo": qlo, "qhi": qhi, "qf": qf, "bufs": [(f, dx, ry, rz) for (f, dx), (ry, rz) in blist]}
+
+
+def lds_bytes(meta: StencilMeta, cfg: KernelConfig, compute: str) -> int:
+    plan = plan_accesses(meta)
+    cb = ELEM_BYTES[compute]
+    tot = 0
+    for f, dx, ry, rz in plan["bufs"]:
+        rzv = -(-rz // cfg.vz) * cfg.vz
+        tot += (cfg.ty + 2 * ry) * (cfg.tz + 2 * rzv) * cb
+    return tot
+
+
+def _arr(vals: Sequence[int]) -> str:
+    vals = list(vals) or [0]
+    return "{" + ", ".join(str(int(v)) for v in vals) + "}"
+
+
+def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str) -> str:
+    plan = plan_accesses(meta)
+    bufs = plan["bufs"]
+    lines = [
+        f"struct P_{tag} {{",
+        f"  using S = {STORAGE_CTYPES[storage]};",
+        f"  using C = {COMPUTE_CTYPES[compute]};",
+        f"  static constexpr int NF = {meta.num_inputs}, NO = {meta.num_outputs};",
+        f"  static constexpr int NT = {cfg.nt}, TY = {cfg.ty}, TZ = {cfg.tz}, VZ = {cfg.vz};",
+        f"  static constexpr int QLO = {plan['qlo']}, QHI = {plan['qhi']};",
+        f"  static constexpr int QF[{max(1, meta.num_inputs)}] = {_arr(plan['qf'])};",
+        f"  static constexpr int NB = {len(bufs)};",
+        f"  static constexpr int BF[{max(1, len(bufs))}] = {_arr([b[0] for b in bufs])};",
+        f"  static constexpr int BDX[{max(1, len(bufs))}] = {_arr([b[1] for b in bufs])};",
+        f"  static constexpr int BRY[{max(1, len(bufs))}] = {_arr([b[2] for b in bufs])};",
+        f"  static constexpr int BRZ[{max(1, len(bufs))}] = {_arr([b[3] for b in bufs])};",
+        "  template <class A> ST_HD static void eval(const A& a, C* out) {",
+    ]
+    for k, e in enumerate(meta.expressions):
+        lines.append(f"    out[{k}] = {e};")
+    lines += ["  }", "};"]
+    return "\n".join(lines)
+
+
+def kernel_symbol(meta: StencilMeta, tag: str) -> str:
+    return f"{meta.name}_sweep_{tag}"
+
+
+def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage: str,
+                    compute: str) -> str:
+    """Full HIP translation unit with one sweep kernel per variant."""
+    out = [
+        f"// {meta.name}: generated by the HIP emitter for gfx950",
+        f"// expression(s): " + " | ".join(meta.py_expressions),
+        '#include "stencil_core.h"',
+        "",
+    ]
+    for cfg in variants:
+        tag = cfg.tag
+        out.append(emit_traits(meta, cfg, storage, compute, tag))
+        out.append(
+            f'extern "C" __global__ void __launch_bounds__({cfg.nt}) '
+            f"{kernel_symbol(meta, tag)}(st::SweepArgs a) {{ st::sweep_entry<P_{tag}>(a); }}"
+        )
+        out.append("")
+    return "\n".join(out)

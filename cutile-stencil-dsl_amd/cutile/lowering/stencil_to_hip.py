@@ -1,0 +1,211 @@
+"""Lowering entry points: stencil IR -> emitted launcher module.
+
+Same entry points and arguments as upstream
+``lower_stencil_to_python`` (``cutile/lowering/stencil_to_cutile.py:250-331``)
+and ``lower_fused_stencils_to_python`` (``cutile/lowering/fusion_emitter.py:
+27-56``); the returned text is a Python module that embeds the emitted HIP
+source and exposes the upstream launcher names:
+
+* ``launch_<name>(u_in, u_out, stream=None)`` (single input),
+  ``launch_<name>(u, v, ..., u_out, stream=None)`` (several inputs),
+  ``launch_<fused>(*inputs, *<member>_out, stream=None)`` (fused);
+* ``apply_boundary_<name>(u, stream=None)`` when the stencil declares a
+  boundary condition;
+* with ``temporal_steps > 1`` one ``launch_<name>`` call advances T steps
+  (intermediate halos zero, as upstream's scratch-buffer chain,
+  ``stencil_to_target.py:740-843``).
+"""
+
+from __future__ import annotations
+
+import json
+from typing import Optional, Sequence
+
+from cutile.frontend.ir import find_func
+from cutile.lowering.emitter import CodeEmitter
+from cutile.lowering.hip_emitter import ELEM_BYTES, KernelConfig, emit_hip_source, kernel_symbol
+from cutile.lowering.meta import StencilMeta, extract_fused_meta, extract_meta
+
+
+def storage_compute(dtype: str, storage: Optional[str]) -> tuple:
+    if storage in (None, dtype):
+        return dtype, dtype
+    if storage == "bfloat16":
+        if dtype != "float32":
+            raise ValueError("bfloat16 storage computes in float32: declare the stencil dtype='float32'")
+        return "bfloat16", "float32"
+    raise ValueError(f"unsupported storage {storage!r} for dtype {dtype!r}")
+
+
+def make_configs(ndim: int, tile: Sequence[int], storage: str) -> tuple:
+    """(vector config, scalar config, chunk) for a ``tile_sizes`` tuple."""
+    tile = tuple(int(t) for t in tile)
+    if ndim == 3:
+        xch, ty, tz = tile
+    elif ndim == 2:
+        (xch, tz), ty = tile, 1
+    else:
+        (tz,), ty, xch = tile, 1, 1
+    sb = ELEM_BYTES[storage]
+    vz = max(1, min(16 // sb, tz))
+
+    def cfg(tz_, vz_):
+        ntz = max(1, tz_ // vz_)
+        if ntz > 1024:
+            raise ValueError(f"tile row of {tz_} elements needs {ntz} threads (> 1024)")
+        if ntz >= 64:
+            nt = ntz * max(1, min(ty, 256 // ntz))
+        else:
+            nt = 64
+        return KernelConfig(ty=ty, tz=tz_, vz=vz_, nt=nt)
+
+    vec = cfg(tz, vz)
+    scal = cfg(max(1, tz // vz), 1)
+    return vec, scal, xch
+
+
+def _attrs(kernel_hints: Optional[dict]) -> str:
+    occ = (kernel_hints or {}).get("occupancy")
+    return f"__attribute__((amdgpu_waves_per_eu({int(occ)}))) " if occ else ""
+
+
+def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Sequence[int],
+                temporal_steps: int = 1, boundary_spec=None, kernel_hints: Optional[dict] = None,
+                storage: Optional[str] = None, fused: bool = False) -> str:
+    storage_t, compute_t = storage_compute(meta.dtype, storage)
+    vec, scal, xch = make_configs(meta.ndim, tile_sizes, storage_t)
+    src = emit_hip_source(meta, [vec, scal], storage_t, compute_t)
+    attr = _attrs(kernel_hints)
+    if attr:
+        src = src.replace('extern "C" __global__ void ', f'extern "C" __global__ void {attr}')
+    variants = {
+        "vec": (kernel_symbol(meta, vec.tag), vec.ty, vec.tz, vec.vz, vec.nt),
+        "scalar": (kernel_symbol(meta, scal.tag), scal.ty, scal.tz, scal.vz, scal.nt),
+    }
+    nd = meta.ndim
+    tvars = ["TX", "TY", "TZ"][:nd]
+    hvars = ["HX", "HY", "HZ"][:nd]
+    tr = "," if nd == 1 else ""
+    multi = meta.num_inputs > 1 or fused
+    inputs = list(meta.input_names) if multi else ["u_in"]
+    outs = meta.output_params() if (fused or meta.num_outputs > 1) else ["u_out"]
+
+    e = CodeEmitter()
+    e.line(f'"""HIP kernel module for the {meta.name} stencil (generated for gfx950)."""')
+    e.blank()
+    e.line("from cutile.runtime.kernel import StencilKernel")
+    e.line("from cutile.runtime import boundary as _boundary")
+    if meta.constants:
+        e.blank()
+        e.line("# Captured constants from stencil definition scope")
+        for k, v in sorted(meta.constants.items()):
+            e.line(f"{k} = {v!r}")
+    e.blank()
+    e.line(f"{', '.join(tvars)}{tr} = {', '.join(str(int(t)) for t in tile_sizes)}")
+    e.line(f"{', '.join(hvars)}{tr} = {', '.join(str(int(h)) for h in halo_widths)}")
+    e.line(f"TEMPORAL_STEPS = {int(temporal_steps)}")
+    e.line(f"STORAGE = {storage_t!r}")
+    e.line(f"INPUTS = {tuple(meta.input_names)!r}")
+    e.line(f"OUTPUTS = {tuple(meta.output_names)!r}")
+    e.line(f"EXPRESSIONS = {meta.py_expressions!r}")
+    e.blank()
+    e.line("_HIP_SOURCE = r'''")
+    for ln in src.splitlines():
+        e.raw(ln)
+    e.raw("'''")
+    e.blank()
+    e.line(f"_VARIANTS = {json.dumps(variants)}")
+    e.line(f"_KERNEL = StencilKernel({meta.name!r}, {nd}, {meta.num_inputs}, {meta.num_outputs}, "
+           f"_HIP_SOURCE, {{k: tuple(v) for k, v in _VARIANTS.items()}}, STORAGE, {int(xch)})")
+    e.line(f"_HALO = ({', '.join(hvars)}{tr})")
+
+    if temporal_steps > 1:
+        e.blank()
+        e.blank()
+        e.line("_TEMPORAL_BUFFERS = {}")
+        e.blank()
+        e.blank()
+        e.line("def _temporal_buffers(ref, count):")
+        with e.indent():
+            e.line('"""`count` reusable scratch arrays shaped like `ref` (zero halos)."""')
+            e.line("import torch")
+            e.line("key = (tuple(ref.shape), str(ref.dtype), ref.device.index)")
+            e.line("bufs = _TEMPORAL_BUFFERS.get(key)")
+            e.line("if bufs is None or len(bufs) < count:")
+            with e.indent():
+                e.line("bufs = [torch.zeros_like(ref) for _ in range(count)]")
+                e.line("_TEMPORAL_BUFFERS[key] = bufs")
+            e.line("return bufs[:count]")
+
+    if boundary_spec is not None and meta.num_outputs == 1 and not fused:
+        e.blank()
+        e.blank()
+        e.line(f"_BOUNDARY = {_spec_repr(boundary_spec)}")
+        e.blank()
+        e.blank()
+        e.line(f"def apply_boundary_{meta.name}(u, stream=None):")
+        with e.indent():
+            e.line('"""Apply boundary conditions to halo cells (on device)."""')
+            e.line("_boundary.fill(u, _BOUNDARY, _HALO, stream)")
+
+    e.blank()
+    e.blank()
+    e.line(f"def launch_{meta.name}({', '.join(inputs + outs)}, stream=None):")
+    with e.indent():
+        ins = f"[{', '.join(inputs)}]"
+        ots = f"[{', '.join(outs)}]"
+        if temporal_steps > 1:
+            e.line(f"# Temporal blocking: {temporal_steps} steps through reusable scratch buffers")
+            e.line(f"bufs = [{inputs[0]}] + _temporal_buffers({inputs[0]}, {temporal_steps - 1}) + [{outs[0]}]")
+            e.line(f"_KERNEL.check_arrays({ins}, {ots})")
+            e.line(f"for _step in range({temporal_steps}):")
+            with e.indent():
+                e.line("_KERNEL.launch([bufs[_step]], [bufs[_step + 1]], _HALO, stream, validate=False)")
+        else:
+            e.line(f"_KERNEL.launch({ins}, {ots}, _HALO, stream)")
+    return e.render()
+
+
+def _spec_repr(spec) -> str:
+    from cutile.frontend.ir import BoundaryAttr
+
+    if isinstance(spec, BoundaryAttr):
+        spec = {"type": spec.bc_type, "value": spec.value}
+    if hasattr(spec, "to_dict"):
+        return repr(spec.to_dict(3))
+    return repr(spec)
+
+
+def lower_stencil_to_python(module, domain=None, tile_sizes=None, halo_widths=None,
+                            temporal_steps: int = 1, boundary_spec=None, kernel_hints=None,
+                            spatial_term_order=None, storage=None) -> str:
+    func = find_func(module)
+    if tile_sizes is None:
+        from cutile.passes.tiling import TilingPass, unique_access_count
+
+        tp = TilingPass(dtype_bytes=4 if func.dtype == "float32" else 8)
+        tile_sizes = tuple(tp.candidates(func.ndim, unique_access_count(func))[0][0])
+    if halo_widths is None:
+        h = func.order // 2 if func.order else 1
+        halo_widths = (h,) * func.ndim
+    meta = extract_meta(func)
+    if boundary_spec is None and meta.boundary is not None:
+        boundary_spec = meta.boundary
+    return emit_module(meta, tile_sizes, halo_widths, temporal_steps, boundary_spec, kernel_hints, storage)
+
+
+def lower_fused_stencils_to_python(modules: Sequence, tile_sizes=None, halo_widths=None,
+                                   temporal_steps: int = 1, kernel_hints=None,
+                                   spatial_term_order=None, storage=None) -> str:
+    if temporal_steps != 1:
+        raise NotImplementedError(
+            "temporal blocking of fused kernels is not supported (temporal_steps must be 1)")
+    meta = extract_fused_meta(modules)
+    if halo_widths is None:
+        halo_widths = (max(1, meta.order // 2),) * meta.ndim
+    if tile_sizes is None:
+        from cutile.passes.tiling import TilingPass
+
+        tp = TilingPass(dtype_bytes=4 if meta.dtype == "float32" else 8)
+        tile_sizes = tuple(tp.tile_for(meta.ndim, len(meta.accesses)))
+    return emit_module(meta, tile_sizes, halo_widths, 1, None, kernel_hints, storage, fused=True)

@@ -1,0 +1,217 @@
+"""Runtime object behind every emitted ``launch_<name>``.
+
+Replaces the cuTile ``ct.launch`` + CuPy glue of the upstream launcher
+(``cutile/lowering/target_to_python.py:305-399``): it JIT-builds the emitted
+HIP source with hiprtc on first use (cached in process and on disk), picks
+the vector or scalar variant from the arrays' alignment, packs the by-value
+``SweepArgs`` block and launches through the C ABI on the caller's stream
+(``None`` = torch's current stream, as upstream used CuPy's current stream,
+``stencil_to_target.py:671-672``).  Launches are asynchronous.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+from typing import Optional, Sequence
+
+from cutile.runtime import rt
+
+MAXF = 12
+
+_TORCH_CODES = {}
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def dtype_code(t) -> int:
+    torch = _torch()
+    if t.dtype == torch.float64:
+        return rt.ST_F64
+    if t.dtype == torch.float32:
+        return rt.ST_F32
+    if t.dtype == torch.bfloat16:
+        return rt.ST_BF16
+    raise TypeError(f"unsupported dtype {t.dtype}")
+
+
+STORAGE_TORCH = {"float64": "float64", "float32": "float32", "bfloat16": "bfloat16"}
+
+
+class SweepArgs(ctypes.Structure):
+    """Mirror of ``st::SweepArgs`` in ``csrc/stencil_core.h``."""
+
+    _fields_ = [
+        ("inp", ctypes.c_void_p * MAXF), ("out", ctypes.c_void_p * MAXF),
+        ("s0", ctypes.c_int64), ("s1", ctypes.c_int64), ("s2", ctypes.c_int64),
+        ("h0", ctypes.c_int), ("h1", ctypes.c_int), ("h2", ctypes.c_int),
+        ("n0", ctypes.c_int), ("n1", ctypes.c_int), ("n2", ctypes.c_int),
+        ("ntz", ctypes.c_int), ("nty", ctypes.c_int), ("nch", ctypes.c_int),
+        ("xch", ctypes.c_int), ("nxcd", ctypes.c_int),
+    ]
+
+
+def map3_shape(ndim: int, shape: Sequence[int]) -> tuple:
+    if ndim == 1:
+        return (1, 1, shape[0])
+    if ndim == 2:
+        return (shape[0], 1, shape[1])
+    return tuple(shape)
+
+
+def default_stream():
+    torch = _torch()
+    return torch.cuda.current_stream().cuda_stream
+
+
+def cache_dir() -> str:
+    d = os.environ.get("CUTILE_AMD_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "cutile_amd", "code"))
+    try:
+        os.makedirs(d, exist_ok=True)
+    except OSError:
+        return ""
+    return d
+
+
+def compile_options(arch: Optional[str] = None) -> list:
+    arch = arch or "gfx950"
+    return [f"--offload-arch={arch}", "-O3", "-std=c++17", "-ffp-contract=off",
+            "-fno-gpu-flush-denormals-to-zero",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            f"-I{rt.CSRC_DIR}"]
+
+
+_arch_cache: dict = {}
+
+
+def current_arch() -> str:
+    dev = _torch().cuda.current_device()
+    if dev not in _arch_cache:
+        _arch_cache[dev] = rt.device_arch(dev)
+    return _arch_cache[dev]
+
+
+def xcd_count(arch: str) -> int:
+    # CDNA3/CDNA4 parts are built from 8 XCDs, each with its own L2
+    return 8 if arch.startswith(("gfx942", "gfx950")) else 1
+
+
+class StencilKernel:
+    """A compiled-on-demand family of sweep variants for one stencil."""
+
+    def __init__(self, name: str, ndim: int, num_inputs: int, num_outputs: int, source: str,
+                 variants: dict, storage: str, xch: int):
+        # variants: {"vec": (symbol, ty, tz, vz, nt), "scalar": (...)}
+        self.name = name
+        self.ndim = ndim
+        self.num_inputs = num_inputs
+        self.num_outputs = num_outputs
+        self.source = source
+        self.variants = variants
+        self.storage = storage
+        self.xch = xch
+        self._modules: dict = {}
+        self._fns: dict = {}
+        self._lock = threading.Lock()
+
+    # -- compilation ------------------------------------------------------
+    def _function(self, key: str):
+        dev = _torch().cuda.current_device()
+        k = (dev, key)
+        fn = self._fns.get(k)
+        if fn is None:
+            with self._lock:
+                mod = self._modules.get(dev)
+                if mod is None:
+                    rt.check(rt.lib().st_init(dev), "st_init")
+                    mod = rt.compile_module(self.source, compile_options(current_arch()), cache_dir(),
+                                            name=f"{self.name}.hip")
+                    self._modules[dev] = mod
+                fn = rt.get_function(mod, self.variants[key][0])
+                self._fns[k] = fn
+        return fn
+
+    def resources(self, key: str = "vec") -> rt.Resources:
+        return rt.function_resources(self._function(key))
+
+    # -- launch -----------------------------------------------------------
+    def check_arrays(self, inputs: Sequence, outputs: Sequence) -> None:
+        torch = _torch()
+        if len(inputs) != self.num_inputs or len(outputs) != self.num_outputs:
+            raise ValueError(f"launch_{self.name}: expected {self.num_inputs} inputs and "
+                             f"{self.num_outputs} outputs, got {len(inputs)} and {len(outputs)}")
+        ref = inputs[0]
+        want = getattr(torch, STORAGE_TORCH[self.storage])
+        for t in list(inputs) + list(outputs):
+            if not isinstance(t, torch.Tensor) or not t.is_cuda:
+                raise TypeError(f"launch_{self.name}: arrays must be torch tensors on a ROCm device")
+            if t.shape != ref.shape:
+                raise ValueError(f"launch_{self.name}: shape {tuple(t.shape)} != {tuple(ref.shape)}")
+            if t.dtype != want:
+                raise TypeError(f"launch_{self.name}: dtype {t.dtype} != {want}")
+            if not t.is_contiguous():
+                raise ValueError(f"launch_{self.name}: arrays must be C-contiguous")
+            if t.dim() != self.ndim:
+                raise ValueError(f"launch_{self.name}: expected {self.ndim}-D arrays")
+        for o in outputs:
+            for i in inputs:
+                if o.data_ptr() == i.data_ptr():
+                    raise ValueError(f"launch_{self.name}: output aliases an input (out-of-place only)")
+
+    def pick(self, inputs, outputs, halo) -> str:
+        s = map3_shape(self.ndim, inputs[0].shape)
+        h = map3_shape(self.ndim, halo) if self.ndim != 1 else (0, 0, halo[0])
+        if self.ndim == 2:
+            h = (halo[0], 0, halo[1])
+        _, _, _, vz, _ = self.variants["vec"]
+        ok = (h[2] % vz == 0) and (s[2] % vz == 0)
+        esz = inputs[0].element_size()
+        ok = ok and all((t.data_ptr() % (vz * esz)) == 0 for t in list(inputs) + list(outputs))
+        return "vec" if ok else "scalar"
+
+    def args(self, inputs, outputs, halo, key: str, xch: Optional[int] = None) -> tuple:
+        _, ty, tz, vz, nt = self.variants[key]
+        s = map3_shape(self.ndim, inputs[0].shape)
+        if self.ndim == 1:
+            h = (0, 0, halo[0])
+        elif self.ndim == 2:
+            h = (halo[0], 0, halo[1])
+        else:
+            h = tuple(halo)
+        n = tuple(a - 2 * b for a, b in zip(s, h))
+        if min(n) <= 0:
+            return None, 0
+        a = SweepArgs()
+        for k, t in enumerate(inputs):
+            a.inp[k] = t.data_ptr()
+        for k, t in enumerate(outputs):
+            a.out[k] = t.data_ptr()
+        a.s0, a.s1, a.s2 = s
+        a.h0, a.h1, a.h2 = h
+        a.n0, a.n1, a.n2 = n
+        x = max(1, min(xch or self.xch, n[0]))
+        a.ntz = -(-n[2] // tz)
+        a.nty = -(-n[1] // ty)
+        a.nch = -(-n[0] // x)
+        a.xch = x
+        a.nxcd = xcd_count(current_arch())
+        return a, a.ntz * a.nty * a.nch
+
+    def launch(self, inputs: Sequence, outputs: Sequence, halo: Sequence[int], stream=None,
+               xch: Optional[int] = None, validate: bool = True) -> None:
+        if validate:
+            self.check_arrays(inputs, outputs)
+        key = self.pick(inputs, outputs, halo)
+        args, nblocks = self.args(inputs, outputs, halo, key, xch)
+        if args is None:
+            return
+        fn = self._function(key)
+        if stream is None:
+            stream = default_stream()
+        rt.launch(fn, (nblocks, 1, 1), (self.variants[key][4], 1, 1), args, stream)
