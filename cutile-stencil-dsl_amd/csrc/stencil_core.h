@@ -75,6 +75,7 @@ struct SweepArgs {
   int ntz, nty, nch;  // tiles along z, y; chunks along x
   int xch;          // planes per chunk
   int nxcd;         // XCDs for the workgroup swizzle (1 = identity)
+  int tmode;        // temporal blocking: 0 = zero halo between levels, 1 = recompute ghosts
 };
 
 // Bijective remap of dispatch order -> logical tile so that XCD x (which
@@ -289,6 +290,126 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// in-kernel temporal blocking (single field): T time levels per launch
+// ---------------------------------------------------------------------------
+//
+// Level 0 is the input array (read through the caches); levels 1..T-1 live
+// in LDS rings of NR = 2*RX+1 planes, level l covering the tile widened by
+// (T-l)*(RY, RZ) so level l+1 can be evaluated on its own region; level T is
+// the output.  The levels are skewed along the streaming axis: at step s,
+// level l produces plane s - l*RX, which reads level l-1 planes
+// s-(l+1)*RX .. s-(l-1)*RX, all resident in the ring.
+//
+// tmode 0 reproduces T back-to-back sweeps through zero-halo scratch arrays:
+// level-l cells outside the interior are 0.  tmode 1 ("recompute") evaluates
+// intermediate levels in the ghost zone too, which equals T steps with a
+// periodic (or any copy-type) refill when the halo is at least T*r deep and
+// was filled before the launch.
+template <class P, int T> struct Temporal {
+  using S = typename P::S;
+  using C = typename P::C;
+  static constexpr int RX = P::RX, RY = P::RY, RZ = P::RZ, NR = 2 * P::RX + 1;
+  static constexpr int NT = P::NT, TY = P::TY, TZ = P::TZ;
+  static constexpr int H(int l) { return TY + 2 * (T - l) * RY; }
+  static constexpr int W(int l) { return TZ + 2 * (T - l) * RZ; }
+  static constexpr int plane(int l) { return H(l) * W(l); }
+  static constexpr int off(int l) { int o = 0; for (int k = 1; k < l; ++k) o += NR * plane(k); return o; }
+  static constexpr int total() { int o = off(T); return o > 0 ? o : 1; }
+
+  static ST_HD int slot(int p) { int r = p % NR; return r < 0 ? r + NR : r; }
+
+  struct GAcc {  // level 0: the input array
+    const S* u; i64 s1, s2, x, y, z;
+    template <int F, int DX, int DY, int DZ> ST_HD C g() const {
+      return Cvt<S, C>::ld(u[((x + DX) * s1 + (y + DY)) * s2 + (z + DZ)]);
+    }
+  };
+  template <int L> struct RAcc {  // level L >= 1 in its LDS ring
+    const C* ring; int p, ly, lz;  // (ly, lz) in level-L region coordinates
+    template <int F, int DX, int DY, int DZ> ST_HD C g() const {
+      return ring[off(L) + slot(p + DX) * plane(L) + (ly + DY) * W(L) + (lz + DZ)];
+    }
+  };
+
+  template <int L>
+  static ST_HD void level(const SweepArgs& a, C* lds, int p, int y0, int z0) {
+    // produce level L (1 <= L < T) plane p over its widened region
+    const int gy0 = y0 - (T - L) * RY, gz0 = z0 - (T - L) * RZ;
+    C* dst = lds + off(L) + slot(p) * plane(L);
+    for (int c = threadIdx.x; c < plane(L); c += NT) {
+      const int ly = c / W(L), lz = c % W(L);
+      const int gy = gy0 + ly, gz = gz0 + lz;
+      bool valid;
+      if (a.tmode == 0) {
+        valid = p >= 0 && p < a.n0 && gy >= 0 && gy < a.n1 && gz >= 0 && gz < a.n2;
+      } else {
+        const i64 ax = (i64)p + a.h0, ay = (i64)gy + a.h1, az = (i64)gz + a.h2;
+        valid = ax >= L * RX && ax < a.s0 - L * RX && ay >= L * RY && ay < a.s1 - L * RY &&
+                az >= L * RZ && az < a.s2 - L * RZ;
+      }
+      C v = C(0);
+      if (valid) {
+        C o[1];
+        if constexpr (L == 1) {
+          GAcc acc{reinterpret_cast<const S*>(a.in[0]), a.s1, a.s2, (i64)p + a.h0, (i64)gy + a.h1,
+                   (i64)gz + a.h2};
+          P::eval(acc, o);
+        } else {
+          RAcc<L - 1> acc{lds, p, ly + RY, lz + RZ};
+          P::eval(acc, o);
+        }
+        v = o[0];
+      }
+      dst[c] = v;
+    }
+  }
+
+  template <int L>
+  static ST_HD void levels(const SweepArgs& a, C* lds, int s, int xb, int xe, int y0, int z0) {
+    if constexpr (L < T) {
+      const int p = s - L * RX;
+      if (p >= xb - (T - L) * RX && p <= xe - 1 + (T - L) * RX) level<L>(a, lds, p, y0, z0);
+      __syncthreads();
+      levels<L + 1>(a, lds, s, xb, xe, y0, z0);
+    }
+  }
+
+  static ST_HD void tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
+    __shared__ C lds[total()];
+    const int z0 = tile_z * TZ, y0 = tile_y * TY;
+    const int xb = chunk * a.xch;
+    const int xe = (xb + a.xch < a.n0) ? xb + a.xch : a.n0;
+    S* out = reinterpret_cast<S*>(a.out[0]);
+    for (int s = xb + (2 - T) * RX; s <= xe - 1 + T * RX; ++s) {
+      levels<1>(a, lds, s, xb, xe, y0, z0);
+      const int p = s - T * RX;
+      if (p >= xb && p < xe) {
+        for (int c = threadIdx.x; c < TY * TZ; c += NT) {
+          const int ly = c / TZ, lz = c % TZ;
+          const int gy = y0 + ly, gz = z0 + lz;
+          if (gy < a.n1 && gz < a.n2) {
+            RAcc<T - 1> acc{lds, p, ly + RY, lz + RZ};
+            C o[1];
+            P::eval(acc, o);
+            out[(((i64)p + a.h0) * a.s1 + (gy + a.h1)) * a.s2 + (gz + a.h2)] = Cvt<S, C>::st(o[0]);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+};
+
+template <class P, int T>
+ST_HD void temporal_entry(const SweepArgs& a) {
+  const int n = a.ntz * a.nty * a.nch;
+  const int t = xcd_swizzle((int)blockIdx.x, n, a.nxcd);
+  const int tile_z = t % a.ntz;
+  const int rest = t / a.ntz;
+  Temporal<P, T>::tile(a, tile_z, rest % a.nty, rest / a.nty);
 }
 
 template <class P>

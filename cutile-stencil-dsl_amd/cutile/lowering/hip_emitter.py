@@ -75,9 +75,34 @@ def _arr(vals: Sequence[int]) -> str:
     return "{" + ", ".join(str(int(v)) for v in vals) + "}"
 
 
+def radius3(meta: StencilMeta) -> tuple:
+    """Stencil radius along each axis of the 3-D (stream, row, column) map."""
+    tab = access_table(meta)
+    return tuple(max([abs(t[k]) for t in tab] + [0]) for k in (1, 2, 3))
+
+
+def temporal_lds_bytes(ty: int, tz: int, T: int, r3: tuple, compute: str) -> int:
+    rx, ry, rz = r3
+    nr = 2 * rx + 1
+    return sum(nr * (ty + 2 * (T - l) * ry) * (tz + 2 * (T - l) * rz) for l in range(1, T)) * ELEM_BYTES[compute]
+
+
+def temporal_tile(ndim: int, T: int, r3: tuple, compute: str, budget: int = 160 * 1024) -> tuple:
+    """Largest in-plane tile whose level rings fit the LDS budget."""
+    if ndim == 3:
+        cands = [(8, 128), (8, 64), (4, 64), (4, 32), (2, 32), (4, 16), (2, 16), (1, 16), (1, 8)]
+    else:
+        cands = [(1, 1024), (1, 512), (1, 256), (1, 128), (1, 64), (1, 32), (1, 16)]
+    for ty, tz in cands:
+        if temporal_lds_bytes(ty, tz, T, r3, compute) <= budget:
+            return ty, tz
+    raise ValueError(f"temporal depth {T} does not fit the LDS budget for radius {r3}")
+
+
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str) -> str:
     plan = plan_accesses(meta)
     bufs = plan["bufs"]
+    r3 = radius3(meta)
     lines = [
         f"struct P_{tag} {{",
         f"  using S = {STORAGE_CTYPES[storage]};",
@@ -91,6 +116,7 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int BDX[{max(1, len(bufs))}] = {_arr([b[1] for b in bufs])};",
         f"  static constexpr int BRY[{max(1, len(bufs))}] = {_arr([b[2] for b in bufs])};",
         f"  static constexpr int BRZ[{max(1, len(bufs))}] = {_arr([b[3] for b in bufs])};",
+        f"  static constexpr int RX = {r3[0]}, RY = {r3[1]}, RZ = {r3[2]};",
         "  template <class A> ST_HD static void eval(const A& a, C* out) {",
     ]
     for k, e in enumerate(meta.expressions):
@@ -103,8 +129,12 @@ def kernel_symbol(meta: StencilMeta, tag: str) -> str:
     return f"{meta.name}_sweep_{tag}"
 
 
+def temporal_symbol(meta: StencilMeta, tag: str, T: int) -> str:
+    return f"{meta.name}_temporal_tb{T}_{tag}"
+
+
 def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage: str,
-                    compute: str) -> str:
+                    compute: str, temporal=None) -> str:
     """Full HIP translation unit with one sweep kernel per variant."""
     out = [
         f"// {meta.name}: generated by the HIP emitter for gfx950",
@@ -118,6 +148,15 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
         out.append(
             f'extern "C" __global__ void __launch_bounds__({cfg.nt}) '
             f"{kernel_symbol(meta, tag)}(st::SweepArgs a) {{ st::sweep_entry<P_{tag}>(a); }}"
+        )
+        out.append("")
+    if temporal is not None:
+        cfg, T = temporal
+        tag = f"tb{T}_{cfg.tag}"
+        out.append(emit_traits(meta, cfg, storage, compute, tag))
+        out.append(
+            f'extern "C" __global__ void __launch_bounds__({cfg.nt}) '
+            f"{temporal_symbol(meta, cfg.tag, T)}(st::SweepArgs a) {{ st::temporal_entry<P_{tag}, {T}>(a); }}"
         )
         out.append("")
     return "\n".join(out)

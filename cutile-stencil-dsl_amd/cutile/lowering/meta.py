@@ -47,12 +47,15 @@ def default_input_names(n: int) -> list:
     return [d[i] if i < len(d) else f"arr{i}" for i in range(n)]
 
 
-def map3(ndim: int, offs: Sequence[int]) -> tuple:
-    """Offsets of an ``ndim``-D access on the 3-D (stream, row, column) map."""
+def map3(ndim: int, offs: Sequence[int], mapping: str = "rows") -> tuple:
+    """Offsets of an ``ndim``-D access on the 3-D (stream, row, column) map.
+
+    2-D arrays stream along their rows by default (``"rows"``: axis 0 is the
+    streaming axis); ``"plane"`` maps them to one plane tiled in 2-D."""
     if ndim == 1:
         return (0, 0, offs[0])
     if ndim == 2:
-        return (offs[0], 0, offs[1])
+        return (0, offs[0], offs[1]) if mapping == "plane" else (offs[0], 0, offs[1])
     return tuple(offs)
 
 
@@ -78,6 +81,7 @@ class StencilMeta:
     constants: dict = field(default_factory=dict)
     boundary: Optional[dict] = None
     flops: int = 0
+    mapping: str = "rows"
 
     @property
     def num_inputs(self) -> int:
@@ -145,8 +149,9 @@ _BIN_CPP = {"add": "+", "sub": "-", "mul": "*", "div": "/"}
 class _Render:
     """Render values of one stencil body as C++ and as Python text."""
 
-    def __init__(self, ndim: int, ctype: str, field_index: dict, view_names: dict):
+    def __init__(self, ndim: int, ctype: str, field_index: dict, view_names: dict, mapping: str = "rows"):
         self.ndim = ndim
+        self.mapping = mapping
         self.ctype = ctype
         self.field_index = field_index      # parameter name -> global input index
         self.view_names = view_names        # Access -> view name
@@ -187,7 +192,7 @@ class _Render:
         if c is not None:
             s = cpp_literal(c, self.ctype)
         elif isinstance(v, Access):
-            dx, dy, dz = map3(self.ndim, v.offsets)
+            dx, dy, dz = map3(self.ndim, v.offsets, self.mapping)
             s = f"a.template g<{self.field_index[v.field]}, {dx}, {dy}, {dz}>()"
         else:
             k = v.kind
@@ -250,7 +255,7 @@ def _view_name(field_name: str, offsets: tuple) -> str:
 
 
 def extract_meta(func: FuncOp, input_names: Optional[Sequence[str]] = None,
-                 compute_dtype: Optional[str] = None) -> StencilMeta:
+                 compute_dtype: Optional[str] = None, mapping: str = "rows") -> StencilMeta:
     """Metadata of a single stencil (``_extract_meta``, ``stencil_to_target.py:163-252``)."""
     meta = StencilMeta(name=func.func_name, ndim=func.ndim, order=func.order, dtype=func.dtype)
     nin = len(func.arg_names)
@@ -271,7 +276,8 @@ def extract_meta(func: FuncOp, input_names: Optional[Sequence[str]] = None,
         names[acc] = _view_name(fname, acc.offsets)
         meta.accesses.append(AccessInfo(acc.index, acc.offsets, names[acc]))
     ctype = "float" if (compute_dtype or func.dtype) == "float32" else "double"
-    r = _Render(func.ndim, ctype, {p: k for k, p in enumerate(func.arg_names)}, names)
+    r = _Render(func.ndim, ctype, {p: k for k, p in enumerate(func.arg_names)}, names, mapping)
+    meta.mapping = mapping
     meta.expressions = [r.cpp(func.result)]
     meta.py_expressions = [r.py(func.result)]
     meta.flops = count_flops(func)
@@ -283,7 +289,8 @@ def fused_name(names: Sequence[str]) -> str:
     return "_".join(names) if len(names) <= 3 else f"{names[0]}_and_{len(names) - 1}_others"
 
 
-def extract_fused_meta(modules: Sequence, compute_dtype: Optional[str] = None) -> StencilMeta:
+def extract_fused_meta(modules: Sequence, compute_dtype: Optional[str] = None,
+                       mapping: str = "rows") -> StencilMeta:
     """Merge several stencils into one multi-output kernel description
     (``extract_fused_meta``, ``stencil_to_target.py:968-1031``)."""
     if not modules:
@@ -314,12 +321,13 @@ def extract_fused_meta(modules: Sequence, compute_dtype: Optional[str] = None) -
             vnames[acc] = vn
             if vn not in seen:
                 seen[vn] = AccessInfo(inputs.index(acc.field), acc.offsets, vn)
-        r = _Render(f.ndim, ctype, {p: inputs.index(p) for p in f.arg_names}, vnames)
+        r = _Render(f.ndim, ctype, {p: inputs.index(p) for p in f.arg_names}, vnames, mapping)
         meta.expressions.append(r.cpp(f.result))
         meta.py_expressions.append(r.py(f.result))
         meta.flops += count_flops(f)
         meta.constants.update(f.constants)
     meta.accesses = list(seen.values())
+    meta.mapping = mapping
     meta.output_names = names
     meta.stencil_inputs = [list(f.arg_names) for f in funcs]
     return meta
@@ -327,4 +335,4 @@ def extract_fused_meta(modules: Sequence, compute_dtype: Optional[str] = None) -
 
 def access_table(meta: StencilMeta) -> list:
     """(field index, dx, dy, dz) of every distinct access on the 3-D map."""
-    return [(a.array_index,) + map3(meta.ndim, a.offsets) for a in meta.accesses]
+    return [(a.array_index,) + map3(meta.ndim, a.offsets, meta.mapping) for a in meta.accesses]

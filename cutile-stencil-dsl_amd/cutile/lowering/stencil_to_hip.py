@@ -12,8 +12,11 @@ source and exposes the upstream launcher names:
 * ``apply_boundary_<name>(u, stream=None)`` when the stencil declares a
   boundary condition;
 * with ``temporal_steps > 1`` one ``launch_<name>`` call advances T steps
-  (intermediate halos zero, as upstream's scratch-buffer chain,
-  ``stencil_to_target.py:740-843``).
+  in one kernel (in-kernel temporal blocking).  ``temporal_mode="zero"``
+  keeps upstream's semantics of T sweeps through zero-halo scratch arrays
+  (``stencil_to_target.py:740-843``); ``"recompute"`` evaluates the
+  intermediate levels inside a halo T*r deep, which equals T steps with a
+  periodic refill when that halo was filled before the call.
 """
 
 from __future__ import annotations
@@ -23,7 +26,8 @@ from typing import Optional, Sequence
 
 from cutile.frontend.ir import find_func
 from cutile.lowering.emitter import CodeEmitter
-from cutile.lowering.hip_emitter import ELEM_BYTES, KernelConfig, emit_hip_source, kernel_symbol
+from cutile.lowering.hip_emitter import (ELEM_BYTES, KernelConfig, emit_hip_source, kernel_symbol,
+                                        plan_accesses, radius3, temporal_symbol, temporal_tile)
 from cutile.lowering.meta import StencilMeta, extract_fused_meta, extract_meta
 
 
@@ -71,10 +75,16 @@ def _attrs(kernel_hints: Optional[dict]) -> str:
 
 def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Sequence[int],
                 temporal_steps: int = 1, boundary_spec=None, kernel_hints: Optional[dict] = None,
-                storage: Optional[str] = None, fused: bool = False) -> str:
+                storage: Optional[str] = None, fused: bool = False, temporal_mode: str = "zero") -> str:
     storage_t, compute_t = storage_compute(meta.dtype, storage)
     vec, scal, xch = make_configs(meta.ndim, tile_sizes, storage_t)
-    src = emit_hip_source(meta, [vec, scal], storage_t, compute_t)
+    temporal = None
+    if temporal_steps > 1:
+        if meta.num_inputs != 1 or meta.num_outputs != 1:
+            raise NotImplementedError("in-kernel temporal blocking needs one input and one output")
+        ty, tz = temporal_tile(meta.ndim, temporal_steps, radius3(meta), compute_t)
+        temporal = (KernelConfig(ty=ty, tz=tz, vz=1, nt=256), int(temporal_steps))
+    src = emit_hip_source(meta, [vec, scal], storage_t, compute_t, temporal)
     attr = _attrs(kernel_hints)
     if attr:
         src = src.replace('extern "C" __global__ void ', f'extern "C" __global__ void {attr}')
@@ -82,6 +92,11 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
         "vec": (kernel_symbol(meta, vec.tag), vec.ty, vec.tz, vec.vz, vec.nt),
         "scalar": (kernel_symbol(meta, scal.tag), scal.ty, scal.tz, scal.vz, scal.nt),
     }
+    if temporal is not None:
+        tcfg, T = temporal
+        variants["temporal"] = (temporal_symbol(meta, tcfg.tag, T), tcfg.ty, tcfg.tz, 1, tcfg.nt)
+    plan = plan_accesses(meta)
+    qdepth = max(1, plan["qhi"] - plan["qlo"] + 1)
     nd = meta.ndim
     tvars = ["TX", "TY", "TZ"][:nd]
     hvars = ["HX", "HY", "HZ"][:nd]
@@ -101,9 +116,12 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
         for k, v in sorted(meta.constants.items()):
             e.line(f"{k} = {v!r}")
     e.blank()
-    e.line(f"{', '.join(tvars)}{tr} = {', '.join(str(int(t)) for t in tile_sizes)}")
-    e.line(f"{', '.join(hvars)}{tr} = {', '.join(str(int(h)) for h in halo_widths)}")
+    e.line(f"{', '.join(tvars)}{tr} = {', '.join(str(int(t)) for t in tile_sizes)}{tr}")
+    e.line(f"{', '.join(hvars)}{tr} = {', '.join(str(int(h)) for h in halo_widths)}{tr}")
     e.line(f"TEMPORAL_STEPS = {int(temporal_steps)}")
+    if temporal_mode not in ("zero", "recompute"):
+        raise ValueError(f"temporal_mode must be 'zero' or 'recompute', got {temporal_mode!r}")
+    e.line(f"TEMPORAL_MODE = {0 if temporal_mode == 'zero' else 1}  # {temporal_mode}")
     e.line(f"STORAGE = {storage_t!r}")
     e.line(f"INPUTS = {tuple(meta.input_names)!r}")
     e.line(f"OUTPUTS = {tuple(meta.output_names)!r}")
@@ -116,26 +134,9 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
     e.blank()
     e.line(f"_VARIANTS = {json.dumps(variants)}")
     e.line(f"_KERNEL = StencilKernel({meta.name!r}, {nd}, {meta.num_inputs}, {meta.num_outputs}, "
-           f"_HIP_SOURCE, {{k: tuple(v) for k, v in _VARIANTS.items()}}, STORAGE, {int(xch)})")
+           f"_HIP_SOURCE, {{k: tuple(v) for k, v in _VARIANTS.items()}}, STORAGE, {int(xch)}, "
+           f"queue_depth={qdepth})")
     e.line(f"_HALO = ({', '.join(hvars)}{tr})")
-
-    if temporal_steps > 1:
-        e.blank()
-        e.blank()
-        e.line("_TEMPORAL_BUFFERS = {}")
-        e.blank()
-        e.blank()
-        e.line("def _temporal_buffers(ref, count):")
-        with e.indent():
-            e.line('"""`count` reusable scratch arrays shaped like `ref` (zero halos)."""')
-            e.line("import torch")
-            e.line("key = (tuple(ref.shape), str(ref.dtype), ref.device.index)")
-            e.line("bufs = _TEMPORAL_BUFFERS.get(key)")
-            e.line("if bufs is None or len(bufs) < count:")
-            with e.indent():
-                e.line("bufs = [torch.zeros_like(ref) for _ in range(count)]")
-                e.line("_TEMPORAL_BUFFERS[key] = bufs")
-            e.line("return bufs[:count]")
 
     if boundary_spec is not None and meta.num_outputs == 1 and not fused:
         e.blank()
@@ -155,12 +156,8 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
         ins = f"[{', '.join(inputs)}]"
         ots = f"[{', '.join(outs)}]"
         if temporal_steps > 1:
-            e.line(f"# Temporal blocking: {temporal_steps} steps through reusable scratch buffers")
-            e.line(f"bufs = [{inputs[0]}] + _temporal_buffers({inputs[0]}, {temporal_steps - 1}) + [{outs[0]}]")
-            e.line(f"_KERNEL.check_arrays({ins}, {ots})")
-            e.line(f"for _step in range({temporal_steps}):")
-            with e.indent():
-                e.line("_KERNEL.launch([bufs[_step]], [bufs[_step + 1]], _HALO, stream, validate=False)")
+            e.line(f"# Temporal blocking: {temporal_steps} steps in one launch (levels in LDS rings)")
+            e.line(f"_KERNEL.launch_temporal({ins}, {ots}, _HALO, stream, mode=TEMPORAL_MODE)")
         else:
             e.line(f"_KERNEL.launch({ins}, {ots}, _HALO, stream)")
     return e.render()
@@ -178,7 +175,7 @@ def _spec_repr(spec) -> str:
 
 def lower_stencil_to_python(module, domain=None, tile_sizes=None, halo_widths=None,
                             temporal_steps: int = 1, boundary_spec=None, kernel_hints=None,
-                            spatial_term_order=None, storage=None) -> str:
+                            spatial_term_order=None, storage=None, temporal_mode: str = "zero") -> str:
     func = find_func(module)
     if tile_sizes is None:
         from cutile.passes.tiling import TilingPass, unique_access_count
@@ -191,7 +188,8 @@ def lower_stencil_to_python(module, domain=None, tile_sizes=None, halo_widths=No
     meta = extract_meta(func)
     if boundary_spec is None and meta.boundary is not None:
         boundary_spec = meta.boundary
-    return emit_module(meta, tile_sizes, halo_widths, temporal_steps, boundary_spec, kernel_hints, storage)
+    return emit_module(meta, tile_sizes, halo_widths, temporal_steps, boundary_spec, kernel_hints, storage,
+                       temporal_mode=temporal_mode)
 
 
 def lower_fused_stencils_to_python(modules: Sequence, tile_sizes=None, halo_widths=None,

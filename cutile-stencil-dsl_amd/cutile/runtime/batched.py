@@ -1,0 +1,258 @@
+"""Batched execution: many instances of one stencil, of uneven sizes, in one
+persistent launch driven by a device tile work-list.
+
+There is no upstream counterpart (upstream launches one kernel per array);
+this is the batching runtime the serving loop and the batched benchmark use.
+
+* Instances are bucketed by size; each bucket has its own tile shape (and,
+  for 2-D, its own mapping: small grids are tiled as one plane so every
+  thread of a workgroup has work, large grids stream along their rows).
+  The kernel contains one sweep instantiation per bucket and dispatches on
+  the work item's bucket.
+* The host builds, once per set of instances, a device table of per-instance
+  launch arguments (``st::SweepArgs``) and a work-list of
+  ``(instance, tile_z, tile_y, chunk|bucket)`` items; a fixed grid of
+  ``occupancy x CUs`` workgroups strides through the list, so the launch
+  shape does not depend on the batch and a captured hipGraph stays valid.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from cutile.runtime import rt
+
+
+@dataclass(frozen=True)
+class Bucket:
+    """Tile configuration for instances whose contiguous extent is at most
+    ``max_cols`` (``None``: unbounded)."""
+
+    mapping: str          # "plane" or "rows" (2-D); "rows" for 3-D
+    ty: int
+    tz: int
+    xch: int = 1
+    max_cols: Optional[int] = None
+
+
+DEFAULT_BUCKETS_2D = (
+    Bucket("plane", 16, 64, 1, 160),
+    Bucket("plane", 8, 128, 1, 384),
+    Bucket("rows", 1, 1024, 16, None),
+)
+
+
+@dataclass
+class BatchPlan:
+    table: object                 # device uint8 tensor holding SweepArgs[]
+    items: object                 # device int32 tensor (n, 4)
+    nitems: int
+    grid: int
+    cells: int
+    instances: int
+    keep: list = field(default_factory=list)   # tensors referenced by the table
+
+
+class BatchedStencil:
+    """One persistent kernel for many arrays of the same stencil."""
+
+    def __init__(self, stencil_fn, storage: Optional[str] = None, buckets: Sequence[Bucket] = None,
+                 occupancy: int = 8):
+        from cutile.frontend.ir import find_func
+        from cutile.lowering.hip_emitter import KernelConfig, emit_traits
+        from cutile.lowering.meta import extract_meta
+        from cutile.lowering.stencil_to_hip import storage_compute
+
+        func = find_func(stencil_fn._ir)
+        self.name = func.func_name
+        self.ndim = func.ndim
+        self.storage, self.compute = storage_compute(func.dtype, storage)
+        esz = {"float64": 8, "float32": 4, "bfloat16": 2}[self.storage]
+        self.vz = 16 // esz
+        if buckets is None:
+            if self.ndim == 2:
+                buckets = [Bucket(b.mapping, b.ty, b.tz * self.vz // 4, b.xch, b.max_cols)
+                           for b in DEFAULT_BUCKETS_2D]
+            else:
+                buckets = [Bucket("rows", 4, 64 * self.vz, 16, None)]
+        self.buckets = list(buckets)
+        self.occupancy = occupancy
+        self.num_inputs = len(func.arg_names)
+        self.inputs = tuple(func.arg_names)
+        parts = ['#include "stencil_core.h"', ""]
+        cases = []
+        self.cfgs = []
+        for k, b in enumerate(self.buckets):
+            meta = extract_meta(func, mapping=b.mapping if self.ndim == 2 else "rows")
+            ntz = b.tz // self.vz
+            if 256 % ntz:
+                raise ValueError(f"bucket {b}: {ntz} threads per row must divide 256")
+            cfg = KernelConfig(ty=b.ty, tz=b.tz, vz=self.vz, nt=256)
+            self.cfgs.append(cfg)
+            parts.append(emit_traits(meta, cfg, self.storage, self.compute, f"b{k}"))
+            cases.append(f"      case {k}: st::sweep_tile<P_b{k}>(a, tz, ty, ch); break;")
+        self.symbol = f"{self.name}_batched"
+        parts.append("struct BatchItem { int inst, tz, ty, cb; };")
+        parts.append(
+            f'extern "C" __global__ void __launch_bounds__(256) {self.symbol}('
+            "const st::SweepArgs* __restrict__ insts, const BatchItem* __restrict__ items, int nitems) {\n"
+            "  for (int i = blockIdx.x; i < nitems; i += gridDim.x) {\n"
+            "    const int inst = __builtin_amdgcn_readfirstlane(items[i].inst);\n"
+            "    const int tz = __builtin_amdgcn_readfirstlane(items[i].tz);\n"
+            "    const int ty = __builtin_amdgcn_readfirstlane(items[i].ty);\n"
+            "    const int cb = __builtin_amdgcn_readfirstlane(items[i].cb);\n"
+            "    const int ch = cb >> 3;\n"
+            "    const st::SweepArgs& a = insts[inst];\n"
+            "    switch (cb & 7) {\n" + "\n".join(cases) + "\n      default: break;\n    }\n"
+            "  }\n}"
+        )
+        self.source = "\n".join(parts)
+        self._fn = {}
+
+    def _function(self):
+        import torch
+
+        from cutile.runtime.kernel import cache_dir, compile_options, current_arch
+
+        dev = torch.cuda.current_device()
+        if dev not in self._fn:
+            mod = rt.compile_module(self.source, compile_options(current_arch()), cache_dir(),
+                                    name=f"{self.symbol}.hip")
+            self._fn[dev] = rt.get_function(mod, self.symbol)
+        return self._fn[dev]
+
+    def resources(self):
+        return rt.function_resources(self._function())
+
+    # ------------------------------------------------------------------
+    def bucket_of(self, n_cols: int) -> int:
+        for k, b in enumerate(self.buckets):
+            if b.max_cols is None or n_cols <= b.max_cols:
+                return k
+        return len(self.buckets) - 1
+
+    def prepare(self, instances: Sequence, halo: Sequence[int], interiors: Optional[Sequence] = None) -> BatchPlan:
+        """``instances``: sequence of ``(inputs, outputs)`` tensor lists, all
+        with the same leading halo; returns the device tables for one launch.
+
+        ``interiors`` gives each instance's interior extents when its arrays
+        are padded beyond the symmetric halo on the high side (rows padded to
+        a multiple of the 16-byte vector keep every row aligned)."""
+        import torch
+
+        from cutile.runtime.kernel import MAXF, SweepArgs, cu_count, xcd_count, current_arch
+
+        nd = self.ndim
+        table = (SweepArgs * max(1, len(instances)))()
+        items: List[tuple] = []
+        cells = 0
+        keep = []
+        for idx, (ins, outs) in enumerate(instances):
+            if len(ins) != self.num_inputs:
+                raise ValueError(f"instance {idx}: expected {self.num_inputs} inputs")
+            shape = tuple(ins[0].shape)
+            inner = tuple(interiors[idx]) if interiors is not None else \
+                tuple(s - 2 * h for s, h in zip(shape, halo))
+            if any(n + 2 * h > s for n, h, s in zip(inner, halo, shape)):
+                raise ValueError(f"instance {idx}: interior {inner} + halo exceeds the array {shape}")
+            if min(inner) <= 0:
+                continue
+            cells += math.prod(inner)
+            k = self.bucket_of(inner[-1])
+            b = self.buckets[k]
+            if nd == 2:
+                if b.mapping == "plane":
+                    s3, h3 = (1, shape[0], shape[1]), (0, halo[0], halo[1])
+                else:
+                    s3, h3 = (shape[0], 1, shape[1]), (halo[0], 0, halo[1])
+            elif nd == 1:
+                s3, h3 = (1, 1, shape[0]), (0, 0, halo[0])
+            else:
+                s3, h3 = shape, tuple(halo)
+            if nd == 2:
+                n3 = (1, inner[0], inner[1]) if b.mapping == "plane" else (inner[0], 1, inner[1])
+            elif nd == 1:
+                n3 = (1, 1, inner[0])
+            else:
+                n3 = tuple(inner)
+            if h3[2] % self.vz or s3[2] % self.vz:
+                raise ValueError(f"instance {idx}: contiguous extent/halo not a multiple of {self.vz}")
+            a = table[idx]
+            for f, t in enumerate(ins):
+                a.inp[f] = t.data_ptr()
+            for m, t in enumerate(outs):
+                a.out[m] = t.data_ptr()
+            for t in list(ins) + list(outs):
+                if t.data_ptr() % 16:
+                    raise ValueError(f"instance {idx}: arrays must be 16-byte aligned")
+            keep += list(ins) + list(outs)
+            a.s0, a.s1, a.s2 = s3
+            a.h0, a.h1, a.h2 = h3
+            a.n0, a.n1, a.n2 = n3
+            a.ntz = -(-n3[2] // b.tz)
+            a.nty = -(-n3[1] // b.ty)
+            a.xch = max(1, min(b.xch, n3[0]))
+            a.nch = -(-n3[0] // a.xch)
+            a.nxcd = 1
+            for ch in range(a.nch):
+                for ty in range(a.nty):
+                    for tz in range(a.ntz):
+                        items.append((idx, tz, ty, ch * 8 + k))
+        blob = bytes(table)
+        dtab = torch.frombuffer(bytearray(blob), dtype=torch.uint8).to("cuda")
+        it = np.asarray(items if items else [(0, 0, 0, 0)], dtype=np.int32).reshape(-1, 4)
+        ditems = torch.from_numpy(it).to("cuda")
+        grid = max(1, min(len(items), self.occupancy * cu_count()))
+        return BatchPlan(dtab, ditems, len(items), grid, cells, len(instances), keep)
+
+    def launch(self, plan: BatchPlan, stream=None) -> None:
+        from cutile.runtime.kernel import default_stream
+
+        if plan.nitems == 0:
+            return
+
+        class _Args(ctypes.Structure):
+            _fields_ = [("insts", ctypes.c_void_p), ("items", ctypes.c_void_p), ("nitems", ctypes.c_int)]
+
+        args = _Args(plan.table.data_ptr(), plan.items.data_ptr(), plan.nitems)
+        rt.launch(self._function(), (plan.grid, 1, 1), (256, 1, 1), args,
+                  stream if stream is not None else default_stream())
+
+
+class LeapfrogBatch:
+    """Two-level (leapfrog) stepping of many instances with one batched
+    stencil ``f(u, uold) -> u_new``: three buffers per instance rotate, so
+    three prepared plans cover every step and a graph of three steps can be
+    replayed indefinitely.  Halos are never written by the sweep, so zero
+    (Dirichlet) halos stay zero."""
+
+    def __init__(self, bs: BatchedStencil, pairs: Sequence, halo: Sequence[int],
+                 interiors: Optional[Sequence] = None):
+        import torch
+
+        self.bs = bs
+        self.halo = tuple(halo)
+        self.bufs = [(u, uold, torch.zeros_like(u)) for u, uold in pairs]
+        rots = [(0, 1, 2), (2, 0, 1), (1, 2, 0)]
+        self.plans = [bs.prepare([([b[i], b[j]], [b[k]]) for b in self.bufs], self.halo, interiors)
+                      for i, j, k in rots]
+        self.steps = 0
+
+    def step(self, stream=None) -> None:
+        self.bs.launch(self.plans[self.steps % 3], stream)
+        self.steps += 1
+
+    def current(self, idx: int):
+        """(u, uold) of instance ``idx`` after the steps taken so far."""
+        b = self.bufs[idx]
+        r = self.steps % 3
+        return [(b[0], b[1]), (b[2], b[0]), (b[1], b[2])][r]
+
+    @property
+    def cells(self) -> int:
+        return self.plans[0].cells

@@ -52,7 +52,7 @@ class SweepArgs(ctypes.Structure):
         ("h0", ctypes.c_int), ("h1", ctypes.c_int), ("h2", ctypes.c_int),
         ("n0", ctypes.c_int), ("n1", ctypes.c_int), ("n2", ctypes.c_int),
         ("ntz", ctypes.c_int), ("nty", ctypes.c_int), ("nch", ctypes.c_int),
-        ("xch", ctypes.c_int), ("nxcd", ctypes.c_int),
+        ("xch", ctypes.c_int), ("nxcd", ctypes.c_int), ("tmode", ctypes.c_int),
     ]
 
 
@@ -97,6 +97,29 @@ def current_arch() -> str:
     return _arch_cache[dev]
 
 
+_cu_cache: dict = {}
+
+
+def cu_count() -> int:
+    dev = _torch().cuda.current_device()
+    if dev not in _cu_cache:
+        _cu_cache[dev] = max(1, rt.device_info(dev).compute_units)
+    return _cu_cache[dev]
+
+
+def chunk_length(n0: int, tiles: int, default: int, queue_depth: int, waves_per_cu: int = 8) -> int:
+    """Planes per workgroup along the streaming axis: the compiled default,
+    shortened until the grid holds about ``waves_per_cu`` workgroups per CU,
+    but not below 4 planes per queue refill so the register-queue prologue
+    (``queue_depth - 1`` planes) stays a small fraction of the chunk."""
+    target = waves_per_cu * cu_count()
+    x = max(1, min(default, n0))
+    if tiles * -(-n0 // x) < target:
+        want = -(-n0 * tiles // target)
+        x = max(min(x, max(4 * max(1, queue_depth - 1), 4)), min(x, want))
+    return max(1, min(x, n0))
+
+
 def xcd_count(arch: str) -> int:
     # CDNA3/CDNA4 parts are built from 8 XCDs, each with its own L2
     return 8 if arch.startswith(("gfx942", "gfx950")) else 1
@@ -106,7 +129,7 @@ class StencilKernel:
     """A compiled-on-demand family of sweep variants for one stencil."""
 
     def __init__(self, name: str, ndim: int, num_inputs: int, num_outputs: int, source: str,
-                 variants: dict, storage: str, xch: int):
+                 variants: dict, storage: str, xch: int, queue_depth: int = 3):
         # variants: {"vec": (symbol, ty, tz, vz, nt), "scalar": (...)}
         self.name = name
         self.ndim = ndim
@@ -116,6 +139,7 @@ class StencilKernel:
         self.variants = variants
         self.storage = storage
         self.xch = xch
+        self.queue_depth = queue_depth
         self._modules: dict = {}
         self._fns: dict = {}
         self._lock = threading.Lock()
@@ -195,13 +219,31 @@ class StencilKernel:
         a.s0, a.s1, a.s2 = s
         a.h0, a.h1, a.h2 = h
         a.n0, a.n1, a.n2 = n
-        x = max(1, min(xch or self.xch, n[0]))
         a.ntz = -(-n[2] // tz)
         a.nty = -(-n[1] // ty)
+        x = xch if xch else chunk_length(n[0], a.ntz * a.nty, self.xch, self.queue_depth)
         a.nch = -(-n[0] // x)
         a.xch = x
         a.nxcd = xcd_count(current_arch())
         return a, a.ntz * a.nty * a.nch
+
+    def launch_temporal(self, inputs: Sequence, outputs: Sequence, halo: Sequence[int], stream=None,
+                        mode: int = 0, xch: Optional[int] = None, validate: bool = True) -> None:
+        """Advance ``T`` steps in one launch (in-kernel temporal blocking).
+        ``mode`` 0: intermediate levels are zero outside the interior (the
+        semantics of T chained sweeps through zero-halo scratch arrays);
+        1: intermediate levels are recomputed in a halo at least T*r deep
+        (T steps with a periodic refill when the halo was filled first)."""
+        if validate:
+            self.check_arrays(inputs, outputs)
+        args, nblocks = self.args(inputs, outputs, halo, "temporal", xch)
+        if args is None:
+            return
+        args.tmode = int(mode)
+        fn = self._function("temporal")
+        if stream is None:
+            stream = default_stream()
+        rt.launch(fn, (nblocks, 1, 1), (self.variants["temporal"][4], 1, 1), args, stream)
 
     def launch(self, inputs: Sequence, outputs: Sequence, halo: Sequence[int], stream=None,
                xch: Optional[int] = None, validate: bool = True) -> None:

@@ -40,6 +40,7 @@ class CompileResult:
     dtype: str = "float64"
     storage: str = "float64"
     hip_source: str = ""
+    temporal_mode: str = "zero"
     _ref_fn: Any = field(default=None, repr=False)
     _ref_fns: Any = field(default=None, repr=False)
     _ref_inputs: Any = field(default=None, repr=False)
@@ -236,12 +237,16 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
             split_axis: int = 0, overlap: bool = True, layout: Optional[str] = None,
             brick_size: int = 32, align_halo: bool = True, occupancy: Optional[int] = None,
             select: Optional[str] = None, temporal_steps: Optional[int] = None,
-            storage: Optional[str] = None, tile_sizes: Optional[tuple] = None) -> CompileResult:
+            storage: Optional[str] = None, tile_sizes: Optional[tuple] = None,
+            temporal_mode: str = "zero") -> CompileResult:
     """Compile a ``@stencil`` function (``launcher.py:342-602``).
 
     Extra keyword arguments of this backend: ``temporal_steps`` forces the
     temporal depth, ``storage="bfloat16"`` stores a float32 stencil's arrays
-    in bfloat16 (arithmetic stays float32), ``tile_sizes`` forces the tile.
+    in bfloat16 (arithmetic stays float32), ``tile_sizes`` forces the tile,
+    ``temporal_mode="recompute"`` (with ``temporal_steps > 1``) widens the
+    halo to ``T * stencil halo`` and recomputes intermediate levels there
+    (T periodic steps per launch after a halo fill).
     ``num_gpus > 1`` is rejected: this backend drives one GPU."""
     from cutile.frontend.ir import ApplyOp
     from cutile.lowering.stencil_to_hip import lower_stencil_to_python
@@ -291,6 +296,8 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
         T = 1  # temporal chains need a single input (upstream breaks there too)
 
     stencil_halo = halo
+    if T > 1 and temporal_mode == "recompute":
+        halo = tuple(T * h for h in halo)
     if align_halo and layout != "bricked":
         halo = pad_inner_halo(halo, dtype_bytes)
 
@@ -327,7 +334,7 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
 
     code = lower_stencil_to_python(ir.clone(), domain=domain, tile_sizes=tiles, halo_widths=halo,
                                    temporal_steps=T, boundary_spec=boundary_spec,
-                                   kernel_hints=hints, storage=storage)
+                                   kernel_hints=hints, storage=storage, temporal_mode=temporal_mode)
     if layout == "bricked":
         from cutile.lowering.bricked import bricked_suffix
 
@@ -337,7 +344,7 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
         name=name, code=code, ndim=stencil_fn._ndim or len(halo), halo_widths=tuple(halo),
         tile_sizes=tuple(tiles), temporal_steps=T, analysis=analysis, stencil_halo=tuple(stencil_halo),
         kernel_hints=hints, resources=resources, inputs=_input_names(ir), dtype=dtype, storage=store,
-        hip_source=src, _ref_fn=stencil_fn._fn, _ref_fns=[stencil_fn._fn],
+        hip_source=src, temporal_mode=temporal_mode, _ref_fn=stencil_fn._fn, _ref_fns=[stencil_fn._fn],
     )
 
 

@@ -191,11 +191,26 @@ class TestCompileAPI:
     def test_temporal_launcher(self):
         r = compile(S(HEAT2D), temporal_steps=3)
         ast.parse(r.code)
-        assert r.temporal_steps == 3 and "range(3)" in r.code and "_temporal_buffers(u_in, 2)" in r.code
+        assert r.temporal_steps == 3 and "TEMPORAL_STEPS = 3" in r.code
+        assert "_KERNEL.launch_temporal([u_in], [u_out]" in r.code
+        assert "st::temporal_entry<P_tb3_" in r.hip_source
+        r2 = compile(S(HEAT2D), temporal_steps=2, temporal_mode="recompute")
+        assert r2.halo_widths == (2, 16) and r2.stencil_halo == (1, 1)
 
     def test_multi_input_launcher_signature(self):
         r = compile(S("def wave(u, uold, i, j):\n    return 2.0 * u[i, j] - uold[i, j]\n"))
         assert "def launch_wave(u, uold, u_out, stream=None):" in r.code
+
+    @pytest.mark.parametrize("src", [
+        "def h1(u, i):\n    return 0.5 * u[i - 1] + 0.5 * u[i + 1]\n", HEAT2D, LAP3D,
+        "def w(u, uold, i, j):\n    return 2.0 * u[i, j] - uold[i, j]\n"])
+    def test_emitted_module_imports(self, src):
+        r = compile(S(src))
+        mod = r.load_module()
+        assert callable(getattr(mod, f"launch_{r.name}"))
+        assert mod._HALO == r.halo_widths
+        r3 = compile(S(src), temporal_steps=2) if "uold" not in src else r
+        assert callable(getattr(r3.load_module(), f"launch_{r3.name}"))
 
     def test_emit_to_file(self, tmp_path):
         r = compile(S(HEAT2D))

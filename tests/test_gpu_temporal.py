@@ -1,0 +1,105 @@
+"""In-kernel temporal blocking and the device time-marching loop (``-m gpu``).
+
+Zero-halo level mode: one launch of depth T equals T chained sweeps through
+zero-halo scratch arrays (the upstream temporal launcher's semantics).
+Recompute mode on a T*r-deep periodic halo, and zero mode for zero
+Dirichlet, equal T steps of (sweep + halo refill).  All comparisons are
+bit-exact against the oracle."""
+
+import numpy as np
+import pytest
+
+from helpers import to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+def _chain_zero_halo(oracle, u, fn, ndim, halo, T):
+    """T sweeps: the first reads u's halo, later ones read zero halos."""
+    cur = u.copy()
+    for t in range(T):
+        nxt = oracle.apply_stencil(cur, fn, ndim, halo)
+        if t < T - 1:
+            z = np.zeros_like(nxt)
+            inner = oracle.interior(nxt.shape, halo)
+            z[inner] = nxt[inner]
+            nxt = z
+        cur = nxt
+    return cur
+
+
+@pytest.mark.parametrize("which,T", [("heat2d", 4), ("diffusion7", 2), ("diffusion7", 3), ("star25", 2)])
+def test_zero_mode_equals_zero_halo_chain(oracle, which, T):
+    import torch
+
+    from cutile import compile
+    from cutile import workloads as w
+
+    fn = getattr(w, which)()
+    r = compile(fn, temporal_steps=T)
+    assert r.temporal_steps == T and "temporal" in r.load_module()._VARIANTS
+    print(f"{which} T={T} tile {r.load_module()._VARIANTS['temporal']}", flush=True)
+    rng = np.random.default_rng(T)
+    base = {2: (70, 150), 3: (23, 19, 45)}[r.ndim]
+    shape = tuple(b + 2 * h for b, h in zip(base, r.halo_widths))
+    u = rng.random(shape)
+    du = to_dev(u)
+    out = torch.zeros_like(du)
+    r.launcher(du, out)
+    ref = _chain_zero_halo(oracle, u, fn._fn, r.ndim, r.halo_widths, T)
+    inner = oracle.interior(shape, r.halo_widths)
+    got = to_host(out)
+    assert np.array_equal(got[inner], ref[inner])
+
+
+@pytest.mark.parametrize("which,T,steps,shape", [
+    ("star25", 2, 5, (24, 20, 36)),
+    ("star25", 4, 6, (20, 24, 40)),
+    ("diffusion7", 4, 9, (30, 18, 50)),
+])
+def test_periodic_recompute_march(oracle, which, T, steps, shape):
+    from cutile import workloads as w
+    from cutile.runtime.timestep import TimeStepper
+
+    fn = getattr(w, which)()
+    spec = w.periodic_spec(3)
+    ts = TimeStepper(fn, spec, temporal_steps=T)
+    assert ts.T == T and ts.mode == "recompute"
+    halo = ts.halo
+    print(f"{which} T={T} halo {halo} steps {steps}", flush=True)
+    u0 = w.sinusoid_init(shape, halo, seed=3)
+    oracle.apply_boundary(u0, spec, halo)
+    got = to_host(ts.march(to_dev(u0), steps))
+    ref = oracle.time_march(u0, fn._fn, 3, halo, steps, spec)
+    assert np.array_equal(got, ref)
+
+
+def test_dirichlet_zero_blocks_match_march(oracle):
+    from cutile import workloads as w
+    from cutile.runtime.timestep import TimeStepper
+
+    fn = w.heat2d()
+    spec = w.dirichlet_spec(2)
+    ts = TimeStepper(fn, spec, temporal_steps=4)
+    assert ts.mode == "zero" and ts.T == 4
+    u0 = w.heat2d_init(100, ts.halo, seed=0)
+    got = to_host(ts.march(to_dev(u0), 11))
+    ref = oracle.time_march(u0, fn._fn, 2, ts.halo, 11, spec)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("T", [1, 2])
+def test_graph_march_matches_eager(oracle, T):
+    from cutile import workloads as w
+    from cutile.runtime.timestep import TimeStepper
+
+    fn = w.diffusion7()
+    spec = w.periodic_spec(3)
+    ts = TimeStepper(fn, spec, temporal_steps=T)
+    u0 = w.sinusoid_init((16, 20, 24), ts.halo, seed=5)
+    oracle.apply_boundary(u0, spec, ts.halo)
+    eager = to_host(ts.march(to_dev(u0), 10))
+    graphed = to_host(ts.march(to_dev(u0), 10, graph=True, graph_steps=4))
+    ref = oracle.time_march(u0, fn._fn, 3, ts.halo, 10, spec)
+    assert np.array_equal(eager, ref)
+    assert np.array_equal(graphed, ref)
