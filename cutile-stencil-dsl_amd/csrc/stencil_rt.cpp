@@ -85,7 +85,7 @@ int do_compile(const char* src, const char* name, const char* const* opts, int n
 
 template <class E>
 __global__ void k_fill_side(E* u, int64_t outer, int64_t n, int64_t inner, int64_t h, int kind,
-                            E value, int side_mask) {
+                            E value, int side_mask, int64_t only_layer) {
   // element e of the (side, k, o, i) slab space
   const int64_t per_side = outer * h * inner;
   const int64_t total = per_side * 2;
@@ -98,6 +98,7 @@ __global__ void k_fill_side(E* u, int64_t outer, int64_t n, int64_t inner, int64
     r %= h * inner;
     const int64_t k = r / inner;
     const int64_t i = r % inner;
+    if (only_layer >= 0 && k != only_layer) continue;
     int64_t dst, src = -1;
     if (side == 0) {
       dst = k;
@@ -112,6 +113,61 @@ __global__ void k_fill_side(E* u, int64_t outer, int64_t n, int64_t inner, int64
     }
     E* base = u + o * n * inner + i;
     base[dst * inner] = (kind == ST_BC_DIRICHLET) ? value : base[src * inner];
+  }
+}
+
+// One element of the two halo slabs of dimension `dim` of a (possibly
+// row-padded) array: e enumerates (side, layer k, other indices) with the
+// other indices in row-major order over the logical shape.
+template <class E>
+__device__ inline void fill_one(E* u, const st_fill_desc& d, int dim, int64_t e) {
+  const int64_t h = d.halo[dim], n = d.shape[dim];
+  int64_t stride[4];
+  stride[d.ndim - 1] = 1;
+  const int64_t rowlen = d.pitch > 0 ? d.pitch : d.shape[d.ndim - 1];
+  for (int k = d.ndim - 2; k >= 0; --k) stride[k] = stride[k + 1] * (k + 1 == d.ndim - 1 ? rowlen : d.shape[k + 1]);
+  // decode: other dims (row-major), then layer k, then side
+  int64_t r = e, off = 0;
+  for (int k = d.ndim - 1; k >= 0; --k) {
+    if (k == dim) continue;
+    off += (r % d.shape[k]) * stride[k];
+    r /= d.shape[k];
+  }
+  const int64_t layer = r % h;
+  const int side = (int)(r / h);
+  const int kind = d.kinds[2 * dim + side];
+  if (kind == ST_BC_NONE) return;
+  int64_t dst, src;
+  if (side == 0) {
+    dst = layer;
+    src = kind == ST_BC_NEUMANN ? h : kind == ST_BC_PERIODIC ? n - 2 * h + layer : 2 * h - 1 - layer;
+  } else {
+    dst = n - h + layer;
+    src = kind == ST_BC_NEUMANN ? n - h - 1 : kind == ST_BC_PERIODIC ? h + layer : n - h - 1 - layer;
+  }
+  E* base = u + off;
+  if (kind == ST_BC_DIRICHLET) {
+    const double v = d.values[2 * dim + side];
+    E ev;
+    if constexpr (sizeof(E) == 2) ev = (E)(unsigned short)v;  // pre-rounded bf16 bits
+    else ev = (E)v;
+    base[dst * stride[dim]] = ev;
+  } else {
+    base[dst * stride[dim]] = base[src * stride[dim]];
+  }
+}
+
+__global__ void k_fill_batched(const st_fill_desc* descs, int dim) {
+  const st_fill_desc& d = descs[blockIdx.y];
+  if (dim >= d.ndim || d.halo[dim] == 0) return;
+  int64_t total = 2 * d.halo[dim];
+  for (int k = 0; k < d.ndim; ++k)
+    if (k != dim) total *= d.shape[k];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (d.dtype == ST_F64) fill_one<double>((double*)d.ptr, d, dim, e);
+    else if (d.dtype == ST_F32) fill_one<float>((float*)d.ptr, d, dim, e);
+    else fill_one<unsigned short>((unsigned short*)d.ptr, d, dim, e);
   }
 }
 
@@ -399,31 +455,50 @@ int st_fill_halo(void* ptr, int dtype, int ndim, const int64_t* shape, const int
     for (int k = d + 1; k < ndim; ++k) inner *= shape[k];
     const int lk = kinds[2 * d], hk = kinds[2 * d + 1];
     // Sides of one dim read/write disjoint layers when the interior is at
-    // least h wide; then one launch per kind does both, else low then high.
-    const bool split = (n - 2 * h) < h || lk != hk || values[2 * d] != values[2 * d + 1];
+    // least h wide; then one launch does both sides, else low then high.
+    // An interior narrower than h lets a side read layers it writes: then
+    // its layers go one launch each, in order, as the sequential definition.
+    const bool narrow = (n - 2 * h) < h;
+    const bool split = narrow || lk != hk || values[2 * d] != values[2 * d + 1];
     for (int pass = 0; pass < (split ? 2 : 1); ++pass) {
       const int kind = split ? (pass == 0 ? lk : hk) : lk;
       const int mask = split ? (1 << pass) : 3;
       const double val = split ? values[2 * d + pass] : values[2 * d];
       if (kind == ST_BC_NONE) continue;
       const unsigned g = grid_for(2 * outer * h * inner);
-      if (dtype == ST_F64) {
-        hipLaunchKernelGGL(k_fill_side<double>, dim3(g), dim3(256), 0, s, (double*)ptr, outer, n,
-                           inner, h, kind, val, mask);
-      } else if (dtype == ST_F32) {
-        hipLaunchKernelGGL(k_fill_side<float>, dim3(g), dim3(256), 0, s, (float*)ptr, outer, n,
-                           inner, h, kind, (float)val, mask);
-      } else {
-        float fv = (float)val;
-        unsigned u;
-        memcpy(&u, &fv, 4);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        hipLaunchKernelGGL(k_fill_side<unsigned short>, dim3(g), dim3(256), 0, s,
-                           (unsigned short*)ptr, outer, n, inner, h, kind,
-                           (unsigned short)(u >> 16), mask);
+      for (int64_t layer = narrow ? 0 : -1; layer < (narrow ? h : 0); ++layer) {
+        if (dtype == ST_F64) {
+          hipLaunchKernelGGL(k_fill_side<double>, dim3(g), dim3(256), 0, s, (double*)ptr, outer, n,
+                             inner, h, kind, val, mask, layer);
+        } else if (dtype == ST_F32) {
+          hipLaunchKernelGGL(k_fill_side<float>, dim3(g), dim3(256), 0, s, (float*)ptr, outer, n,
+                             inner, h, kind, (float)val, mask, layer);
+        } else {
+          float fv = (float)val;
+          unsigned u;
+          memcpy(&u, &fv, 4);
+          u += 0x7fffu + ((u >> 16) & 1u);
+          hipLaunchKernelGGL(k_fill_side<unsigned short>, dim3(g), dim3(256), 0, s,
+                             (unsigned short*)ptr, outer, n, inner, h, kind,
+                             (unsigned short)(u >> 16), mask, layer);
+        }
+        HIPCHK(hipGetLastError());
       }
-      HIPCHK(hipGetLastError());
     }
+  }
+  return ST_OK;
+}
+
+int st_fill_halo_batched(const void* descs, int count, int ndim, int64_t max_slab, void* stream) {
+  if (!descs || count < 0 || ndim < 1 || ndim > 4) return fail(ST_E_INVALID, "bad fill_halo_batched arguments");
+  if (count == 0) return ST_OK;
+  if (count > 65535) return fail(ST_E_INVALID, "too many arrays in one batched fill (%d)", count);
+  unsigned gx = grid_for(max_slab);
+  if (gx > 1024) gx = 1024;
+  for (int d = 0; d < ndim; ++d) {
+    hipLaunchKernelGGL(k_fill_batched, dim3(gx, (unsigned)count), dim3(256), 0, (hipStream_t)stream,
+                       (const st_fill_desc*)descs, d);
+    HIPCHK(hipGetLastError());
   }
   return ST_OK;
 }

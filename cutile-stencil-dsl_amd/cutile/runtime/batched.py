@@ -65,11 +65,21 @@ class BatchedStencil:
                  occupancy: int = 8):
         from cutile.frontend.ir import find_func
         from cutile.lowering.hip_emitter import KernelConfig, emit_traits
-        from cutile.lowering.meta import extract_meta
+        from cutile.lowering.meta import extract_fused_meta, extract_meta
         from cutile.lowering.stencil_to_hip import storage_compute
 
-        func = find_func(stencil_fn._ir)
-        self.name = func.func_name
+        fns = list(stencil_fn) if isinstance(stencil_fn, (list, tuple)) else [stencil_fn]
+        funcs = [find_func(f._ir) for f in fns]
+        func = funcs[0]
+        if len(fns) > 1:
+            probe = extract_fused_meta([f._ir for f in fns])
+            self.name = probe.name
+            self.inputs = tuple(probe.input_names)
+            self.outputs = tuple(probe.output_names)
+        else:
+            self.name = func.func_name
+            self.inputs = tuple(func.arg_names)
+            self.outputs = (func.func_name,)
         self.ndim = func.ndim
         self.storage, self.compute = storage_compute(func.dtype, storage)
         esz = {"float64": 8, "float32": 4, "bfloat16": 2}[self.storage]
@@ -82,13 +92,15 @@ class BatchedStencil:
                 buckets = [Bucket("rows", 4, 64 * self.vz, 16, None)]
         self.buckets = list(buckets)
         self.occupancy = occupancy
-        self.num_inputs = len(func.arg_names)
-        self.inputs = tuple(func.arg_names)
+        self.num_inputs = len(self.inputs)
+        self.num_outputs = len(self.outputs)
         parts = ['#include "stencil_core.h"', ""]
         cases = []
         self.cfgs = []
         for k, b in enumerate(self.buckets):
-            meta = extract_meta(func, mapping=b.mapping if self.ndim == 2 else "rows")
+            mp = b.mapping if self.ndim == 2 else "rows"
+            meta = (extract_fused_meta([f._ir for f in fns], mapping=mp) if len(fns) > 1
+                    else extract_meta(func, mapping=mp))
             ntz = b.tz // self.vz
             if 256 % ntz:
                 raise ValueError(f"bucket {b}: {ntz} threads per row must divide 256")
@@ -153,8 +165,9 @@ class BatchedStencil:
         cells = 0
         keep = []
         for idx, (ins, outs) in enumerate(instances):
-            if len(ins) != self.num_inputs:
-                raise ValueError(f"instance {idx}: expected {self.num_inputs} inputs")
+            if len(ins) != self.num_inputs or len(outs) != self.num_outputs:
+                raise ValueError(f"instance {idx}: expected {self.num_inputs} inputs and "
+                                 f"{self.num_outputs} outputs")
             shape = tuple(ins[0].shape)
             inner = tuple(interiors[idx]) if interiors is not None else \
                 tuple(s - 2 * h for s, h in zip(shape, halo))
@@ -199,16 +212,20 @@ class BatchedStencil:
             a.xch = max(1, min(b.xch, n3[0]))
             a.nch = -(-n3[0] // a.xch)
             a.nxcd = 1
-            for ch in range(a.nch):
-                for ty in range(a.nty):
-                    for tz in range(a.ntz):
-                        items.append((idx, tz, ty, ch * 8 + k))
+            ch, ty, tz = np.meshgrid(np.arange(a.nch), np.arange(a.nty), np.arange(a.ntz), indexing="ij")
+            blk = np.empty((ch.size, 4), dtype=np.int32)
+            blk[:, 0] = idx
+            blk[:, 1] = tz.ravel()
+            blk[:, 2] = ty.ravel()
+            blk[:, 3] = ch.ravel() * 8 + k
+            items.append(blk)
         blob = bytes(table)
         dtab = torch.frombuffer(bytearray(blob), dtype=torch.uint8).to("cuda")
-        it = np.asarray(items if items else [(0, 0, 0, 0)], dtype=np.int32).reshape(-1, 4)
-        ditems = torch.from_numpy(it).to("cuda")
-        grid = max(1, min(len(items), self.occupancy * cu_count()))
-        return BatchPlan(dtab, ditems, len(items), grid, cells, len(instances), keep)
+        it = np.concatenate(items) if items else np.zeros((1, 4), dtype=np.int32)
+        ditems = torch.from_numpy(np.ascontiguousarray(it)).to("cuda")
+        nitems = int(sum(len(b) for b in items))
+        grid = max(1, min(nitems, self.occupancy * cu_count()))
+        return BatchPlan(dtab, ditems, nitems, grid, cells, len(instances), keep)
 
     def launch(self, plan: BatchPlan, stream=None) -> None:
         from cutile.runtime.kernel import default_stream

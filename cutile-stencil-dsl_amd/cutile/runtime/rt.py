@@ -24,7 +24,8 @@ BC_CODES = {None: 0, "none": 0, "dirichlet": 1, "neumann": 2, "periodic": 3, "re
 EXPORTS = (
     "st_last_error", "st_init", "st_device_count", "st_device_info_get", "st_mem_info",
     "st_compile_only", "st_free", "st_compile", "st_load_code", "st_unload", "st_get_function",
-    "st_launch", "st_function_resources", "st_fill_halo", "st_copy", "st_diff_norms", "st_bricks",
+    "st_launch", "st_function_resources", "st_fill_halo", "st_fill_halo_batched", "st_copy", "st_diff_norms",
+    "st_bricks",
     "st_event_create", "st_event_record", "st_event_elapsed_ms", "st_event_destroy",
     "st_stream_sync", "st_device_sync", "st_graph_begin", "st_graph_end", "st_graph_launch",
     "st_graph_destroy",
@@ -44,6 +45,14 @@ class DeviceInfo(ctypes.Structure):
         ("mem_clock_khz", ctypes.c_int), ("mem_bus_width", ctypes.c_int),
         ("warp_size", ctypes.c_int),
     ]
+
+
+class FillDesc(ctypes.Structure):
+    """Mirror of ``st_fill_desc``."""
+
+    _fields_ = [("ptr", ctypes.c_void_p), ("shape", ctypes.c_int64 * 4), ("halo", ctypes.c_int64 * 4),
+                ("kinds", ctypes.c_int * 8), ("values", ctypes.c_double * 8), ("ndim", ctypes.c_int),
+                ("dtype", ctypes.c_int), ("pitch", ctypes.c_int64)]
 
 
 class Resources(ctypes.Structure):
@@ -75,6 +84,7 @@ def _declare(lib):
         "st_function_resources": ([vp, ctypes.POINTER(Resources)], ctypes.c_int),
         "st_fill_halo": ([vp, ctypes.c_int, ctypes.c_int, i64p, i64p, ctypes.POINTER(ctypes.c_int),
                           ctypes.POINTER(ctypes.c_double), vp], ctypes.c_int),
+        "st_fill_halo_batched": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, vp], ctypes.c_int),
         "st_copy": ([vp, vp, ctypes.c_size_t, vp], ctypes.c_int),
         "st_diff_norms": ([vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, i64p, i64p, vp,
                            ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
@@ -213,6 +223,10 @@ def fill_halo(ptr: int, dtype_code: int, shape, halo, kinds, values, stream=None
     k = (ctypes.c_int * (2 * nd))(*kinds)
     v = (ctypes.c_double * (2 * nd))(*values)
     check(lib().st_fill_halo(ptr, dtype_code, nd, _i64(shape), _i64(halo), k, v, stream), "st_fill_halo")
+
+
+def fill_halo_batched(descs_dev_ptr: int, count: int, ndim: int, max_slab: int, stream=None) -> None:
+    check(lib().st_fill_halo_batched(descs_dev_ptr, count, ndim, max_slab, stream), "st_fill_halo_batched")
 
 
 def copy(dst: int, src: int, nbytes: int, stream=None) -> None:

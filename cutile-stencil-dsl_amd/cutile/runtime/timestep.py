@@ -121,6 +121,18 @@ class TimeStepper:
         self.fill(u, stream)
         if not graph:
             return self.enqueue(u, b, steps, stream)[0]
+        # graphs are captured on a private stream ordered after the caller's
+        cur = torch.cuda.current_stream()
+        side = torch.cuda.Stream()
+        side.wait_stream(cur)
+        with torch.cuda.stream(side):
+            out = self._march_graph(u, b, steps, side.cuda_stream, graph_steps)
+        cur.wait_stream(side)
+        return out
+
+    def _march_graph(self, u, b, steps, stream, graph_steps):
+        import torch
+
         # warm the JIT outside the capture
         self._single._function(self._single.pick([u], [b], self.halo))
         if self._block is not None:

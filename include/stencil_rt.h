@@ -117,6 +117,25 @@ int st_function_resources(void* function, st_resources* out);
 int st_fill_halo(void* ptr, int dtype, int ndim, const int64_t* shape, const int64_t* halo,
                  const int* kinds, const double* values, void* stream);
 
+/* One halo-fill descriptor of a batch (device-resident array of these). */
+typedef struct st_fill_desc {
+  void* ptr;
+  int64_t shape[4];
+  int64_t halo[4];
+  int kinds[8];       /* low0, high0, low1, high1, ... */
+  double values[8];
+  int ndim;
+  int dtype;
+  int64_t pitch;      /* row stride (elements) of the last axis; 0 = shape[ndim-1] */
+} st_fill_desc;
+
+/* Halo fill of `count` arrays in one launch per dimension (serving and
+ * batched stepping; no upstream counterpart).  `descs` is DEVICE memory
+ * holding `count` st_fill_desc; every array must have an interior at least
+ * as wide as its halo in each dimension.  `max_slab` bounds the elements of
+ * any one array's two slabs in any dimension (sizes the grid). */
+int st_fill_halo_batched(const void* descs, int count, int ndim, int64_t max_slab, void* stream);
+
 /* dst[i] = src[i] for nbytes bytes (16-byte vector copy kernel). */
 int st_copy(void* dst, const void* src, size_t nbytes, void* stream);
 
