@@ -76,6 +76,7 @@ struct SweepArgs {
   int xch;          // planes per chunk
   int nxcd;         // XCDs for the workgroup swizzle (1 = identity)
   int tmode;        // temporal blocking: 0 = zero halo between levels, 1 = recompute ghosts
+  int yslab;        // tile order: 0 = XCD gets an x-slab (z, y, x-chunk), 1 = a y-slab (z, x-chunk, y)
 };
 
 // Bijective remap of dispatch order -> logical tile so that XCD x (which
@@ -91,18 +92,42 @@ template <class P> struct Tab {
   static constexpr int NTZ = P::TZ / P::VZ;
   static constexpr int NTY = P::NT / NTZ;
   static constexpr int KY = (P::TY + NTY - 1) / NTY;
-  static constexpr int QN = (P::QHI >= P::QLO) ? (P::QHI - P::QLO + 1) : 1;
+  static constexpr bool HASQ = P::QHI >= P::QLO;
+  static constexpr int QN = HASQ ? (P::QHI - P::QLO + 1) : 1;
   static constexpr int rzv(int b) { return ((P::BRZ[b] + P::VZ - 1) / P::VZ) * P::VZ; }
   static constexpr int rows(int b) { return P::TY + 2 * P::BRY[b]; }
   static constexpr int cols(int b) { return P::TZ + 2 * rzv(b); }
-  static constexpr int vecs(int b) { return rows(b) * cols(b) / P::VZ; }
-  static constexpr int nvb(int b) { return (vecs(b) + P::NT - 1) / P::NT; }
+  static constexpr int vpr(int b) { return cols(b) / P::VZ; }
+  static constexpr int svec(int b) { return rzv(b) / P::VZ; }
+  // the tile interior of buffer b is copied from the register queue (only
+  // the halo ring is fetched from memory)
+  static constexpr bool fromq(int b) {
+    return HASQ && P::QF[P::BF[b]] && P::BDX[b] >= P::QLO && P::BDX[b] <= P::QHI;
+  }
+  static constexpr int gvecs(int b) {
+    return fromq(b) ? 2 * P::BRY[b] * vpr(b) + P::TY * 2 * svec(b) : rows(b) * vpr(b);
+  }
+  static constexpr int nvb(int b) { return (gvecs(b) + P::NT - 1) / P::NT; }
   static constexpr int off(int b) { int o = 0; for (int k = 0; k < b; ++k) o += rows(k) * cols(k); return o; }
   static constexpr int lds_elems() { int o = 0; for (int k = 0; k < P::NB; ++k) o += rows(k) * cols(k); return o > 0 ? o : 1; }
   static constexpr int max_nvb() { int m = 1; for (int k = 0; k < P::NB; ++k) m = nvb(k) > m ? nvb(k) : m; return m; }
   static constexpr int buf(int f, int dx) {
     for (int k = 0; k < P::NB; ++k) if (P::BF[k] == f && P::BDX[k] == dx) return k;
     return -1;
+  }
+  // (row, vector column) in buffer b of the i-th vector fetched from memory
+  static ST_HD void gvec_rc(int b, int i, int& r, int& c) {
+    const int w = vpr(b);
+    if (!fromq(b)) { r = i / w; c = i % w; return; }
+    const int band = P::BRY[b] * w;
+    if (i < band) { r = i / w; c = i % w; return; }
+    i -= band;
+    if (i < band) { r = P::BRY[b] + P::TY + i / w; c = i % w; return; }
+    i -= band;
+    const int sv = svec(b);
+    r = P::BRY[b] + i / (2 * sv);
+    const int k = i % (2 * sv);
+    c = k < sv ? k : w - 2 * sv + k;
   }
 };
 
@@ -115,7 +140,7 @@ template <class P> struct Cell {
   int k, v, ly, lz;
 
   template <int F, int DX, int DY, int DZ> ST_HD C g() const {
-    if constexpr (DY == 0 && DZ == 0 && P::QHI >= P::QLO && DX >= P::QLO && DX <= P::QHI) {
+    if constexpr (DY == 0 && DZ == 0 && T::HASQ && DX >= P::QLO && DX <= P::QHI) {
       return q[F][DX - P::QLO][k][v];
     } else {
       constexpr int b = T::buf(F, DX);
@@ -124,6 +149,9 @@ template <class P> struct Cell {
     }
   }
 };
+
+template <class S, int V> struct NtVec { typedef S type __attribute__((ext_vector_type(V))); };
+template <class S> struct NtVec<S, 1> { typedef S type; };
 
 // ---------------------------------------------------------------------------
 // the plane-streaming sweep
@@ -145,8 +173,9 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   const int xb = chunk * a.xch;
   const int xe = (xb + a.xch < a.n0) ? xb + a.xch : a.n0;
   const i64 s1 = a.s1, s2 = a.s2;
-  const int zc = z0 + tz * VZ;                 // first interior z of my vector
-  const bool zin = zc < a.n2;
+  const int zc = z0 + tz * VZ;                  // first interior z of my vector
+  const bool zin = zc < a.n2;                   // my cells are interior (stored)
+  const bool zload = a.h2 + zc + VZ <= s2;      // my vector lies inside the array
 
   C q[NF][QN][KY][VZ];
   C pq[NF][KY][VZ];
@@ -156,13 +185,14 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     return reinterpret_cast<const S*>(a.in[f]) + (i64)(a.h0 + x) * s1 * s2;
   };
 
-  // load my cells of plane x of field f into dst
+  // my cells of plane x of field f (cells outside the interior but inside
+  // the array are loaded too: they are the halo other cells read)
   auto load_cells = [&](int f, int x, C (&dst)[KY][VZ]) {
     const S* p = plane_ptr(f, x);
 #pragma unroll
     for (int k = 0; k < KY; ++k) {
       const int y = y0 + ty + k * NTY;
-      if (zin && y < a.n1 && ty + k * NTY < TY) {
+      if (zload && a.h1 + y < s1 && ty + k * NTY < TY) {
         Vec<S, VZ> w = vload<S, VZ>(p + (i64)(a.h1 + y) * s2 + a.h2 + zc);
 #pragma unroll
         for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
@@ -173,17 +203,17 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     }
   };
 
-  // prefetch the LDS vectors of plane x (tile + halo) for every buffer
+  // prefetch the memory part (halo ring, or whole tile) of every LDS buffer
   auto load_bufs = [&](int x) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const S* p = plane_ptr(P::BF[b], x + P::BDX[b]);
-      const int vpr = T::cols(b) / VZ;
 #pragma unroll
       for (int j = 0; j < NVB; ++j) {
         const int i = tid + j * NT;
-        bool ok = (j < T::nvb(b)) && (i < T::vecs(b));
-        const int r = i / vpr, c = i % vpr;
+        bool ok = (j < T::nvb(b)) && (i < T::gvecs(b));
+        int r = 0, c = 0;
+        T::gvec_rc(b, ok ? i : 0, r, c);
         const i64 gy = (i64)a.h1 + y0 - P::BRY[b] + r;
         const i64 gz = (i64)a.h2 + z0 - T::rzv(b) + c * VZ;
         ok = ok && gy >= 0 && gy < s1 && gz >= 0 && gz + VZ <= s2;
@@ -202,11 +232,26 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   auto store_bufs = [&]() {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
+      C* base = lds + T::off(b);
+      if constexpr (T::HASQ) {
+        if (T::fromq(b)) {
+#pragma unroll
+          for (int k = 0; k < KY; ++k) {
+            if (ty + k * NTY < TY) {
+              C* d = base + (P::BRY[b] + ty + k * NTY) * T::cols(b) + T::rzv(b) + tz * VZ;
+#pragma unroll
+              for (int v = 0; v < VZ; ++v) d[v] = q[P::BF[b]][P::BDX[b] - P::QLO][k][v];
+            }
+          }
+        }
+      }
 #pragma unroll
       for (int j = 0; j < NVB; ++j) {
         const int i = tid + j * NT;
-        if (j < T::nvb(b) && i < T::vecs(b)) {
-          C* d = lds + T::off(b) + i * VZ;
+        if (j < T::nvb(b) && i < T::gvecs(b)) {
+          int r, c;
+          T::gvec_rc(b, i, r, c);
+          C* d = base + r * T::cols(b) + c * VZ;
 #pragma unroll
           for (int v = 0; v < VZ; ++v) d[v] = pb[b][j][v];
         }
@@ -215,7 +260,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   };
 
   // ---- prologue: queue planes xb+QLO .. xb+QHI-1, prefetch plane xb+QHI
-  if constexpr (P::QHI >= P::QLO) {
+  if constexpr (T::HASQ) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       if (!P::QF[f]) continue;
@@ -228,7 +273,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
 
   for (int x = xb; x < xe; ++x) {
     // install plane x+QHI at the back of the queue
-    if constexpr (P::QHI >= P::QLO) {
+    if constexpr (T::HASQ) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         if (!P::QF[f]) continue;
@@ -251,7 +296,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     }
     // prefetch plane x+1 while plane x is computed
     if (x + 1 < xe) {
-      if constexpr (P::QHI >= P::QLO) {
+      if constexpr (T::HASQ) {
 #pragma unroll
         for (int f = 0; f < NF; ++f)
           if (P::QF[f]) load_cells(f, x + 1 + P::QHI, pq[f]);
@@ -278,10 +323,18 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       for (int m = 0; m < NO; ++m) {
         S* dst = reinterpret_cast<S*>(a.out[m]) + off;
         if (zc + VZ <= a.n2) {
-          Vec<S, VZ> w;
+          typename NtVec<S, VZ>::type w;
+          if constexpr (VZ == 1) {
+            w = Cvt<S, C>::st(res[m][0]);
+          } else {
 #pragma unroll
-          for (int v = 0; v < VZ; ++v) w.v[v] = Cvt<S, C>::st(res[m][v]);
-          vstore<S, VZ>(dst, w);
+            for (int v = 0; v < VZ; ++v) w[v] = Cvt<S, C>::st(res[m][v]);
+          }
+          if constexpr (P::NTS) {
+            __builtin_nontemporal_store(w, reinterpret_cast<typename NtVec<S, VZ>::type*>(dst));
+          } else {
+            *reinterpret_cast<typename NtVec<S, VZ>::type*>(dst) = w;
+          }
         } else {
 #pragma unroll
           for (int v = 0; v < VZ; ++v)
@@ -418,9 +471,11 @@ ST_HD void sweep_entry(const SweepArgs& a) {
   const int t = xcd_swizzle((int)blockIdx.x, n, a.nxcd);
   const int tile_z = t % a.ntz;
   const int rest = t / a.ntz;
-  const int tile_y = rest % a.nty;
-  const int chunk = rest / a.nty;
-  sweep_tile<P>(a, tile_z, tile_y, chunk);
+  if (a.yslab) {
+    sweep_tile<P>(a, tile_z, rest / a.nch, rest % a.nch);
+  } else {
+    sweep_tile<P>(a, tile_z, rest % a.nty, rest / a.nty);
+  }
 }
 
 }  // namespace st

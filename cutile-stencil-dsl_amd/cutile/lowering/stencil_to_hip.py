@@ -69,7 +69,7 @@ def make_configs(ndim: int, tile: Sequence[int], storage: str) -> tuple:
 
 
 def _attrs(kernel_hints: Optional[dict]) -> str:
-    occ = (kernel_hints or {}).get("occupancy")
+    occ = (kernel_hints or {}).get("occupancy")  # other hints ("nontemporal") are emitter options
     return f"__attribute__((amdgpu_waves_per_eu({int(occ)}))) " if occ else ""
 
 
@@ -84,7 +84,8 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
             raise NotImplementedError("in-kernel temporal blocking needs one input and one output")
         ty, tz = temporal_tile(meta.ndim, temporal_steps, radius3(meta), compute_t)
         temporal = (KernelConfig(ty=ty, tz=tz, vz=1, nt=256), int(temporal_steps))
-    src = emit_hip_source(meta, [vec, scal], storage_t, compute_t, temporal)
+    src = emit_hip_source(meta, [vec, scal], storage_t, compute_t, temporal,
+                          nontemporal=bool((kernel_hints or {}).get("nontemporal", 1)))
     attr = _attrs(kernel_hints)
     if attr:
         src = src.replace('extern "C" __global__ void ', f'extern "C" __global__ void {attr}')

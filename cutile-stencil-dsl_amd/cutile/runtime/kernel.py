@@ -53,6 +53,7 @@ class SweepArgs(ctypes.Structure):
         ("n0", ctypes.c_int), ("n1", ctypes.c_int), ("n2", ctypes.c_int),
         ("ntz", ctypes.c_int), ("nty", ctypes.c_int), ("nch", ctypes.c_int),
         ("xch", ctypes.c_int), ("nxcd", ctypes.c_int), ("tmode", ctypes.c_int),
+        ("yslab", ctypes.c_int),
     ]
 
 
@@ -199,6 +200,8 @@ class StencilKernel:
         ok = ok and all((t.data_ptr() % (vz * esz)) == 0 for t in list(inputs) + list(outputs))
         return "vec" if ok else "scalar"
 
+    yslab = 0
+
     def args(self, inputs, outputs, halo, key: str, xch: Optional[int] = None) -> tuple:
         _, ty, tz, vz, nt = self.variants[key]
         s = map3_shape(self.ndim, inputs[0].shape)
@@ -225,6 +228,7 @@ class StencilKernel:
         a.nch = -(-n[0] // x)
         a.xch = x
         a.nxcd = xcd_count(current_arch())
+        a.yslab = self.yslab
         return a, a.ntz * a.nty * a.nch
 
     def launch_temporal(self, inputs: Sequence, outputs: Sequence, halo: Sequence[int], stream=None,
