@@ -177,8 +177,9 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   const bool zin = zc < a.n2;                   // my cells are interior (stored)
   const bool zload = a.h2 + zc + VZ <= s2;      // my vector lies inside the array
 
+  constexpr int PD = P::PD;   // planes prefetched ahead into registers
   C q[NF][QN][KY][VZ];
-  C pq[NF][KY][VZ];
+  C pq[PD][NF][KY][VZ];
   C pb[NB > 0 ? NB : 1][NVB][VZ];
 
   auto plane_ptr = [&](int f, int x) -> const S* {
@@ -259,14 +260,17 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     }
   };
 
-  // ---- prologue: queue planes xb+QLO .. xb+QHI-1, prefetch plane xb+QHI
+  // ---- prologue: queue planes xb+QLO .. xb+QHI-1, prefetch planes
+  //      xb+QHI .. xb+QHI+PD-1 (those the chunk will install)
   if constexpr (T::HASQ) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       if (!P::QF[f]) continue;
 #pragma unroll
       for (int i = 0; i + 1 < QN; ++i) load_cells(f, xb + P::QLO + i, q[f][i + 1]);
-      load_cells(f, xb + P::QHI, pq[f]);
+#pragma unroll
+      for (int k = 0; k < PD; ++k)
+        if (xb + k < xe) load_cells(f, xb + P::QHI + k, pq[k][f]);
     }
   }
   load_bufs(xb);
@@ -286,7 +290,13 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
 #pragma unroll
         for (int k = 0; k < KY; ++k)
 #pragma unroll
-          for (int v = 0; v < VZ; ++v) q[f][QN - 1][k][v] = pq[f][k][v];
+          for (int v = 0; v < VZ; ++v) q[f][QN - 1][k][v] = pq[0][f][k][v];
+#pragma unroll
+        for (int d = 0; d + 1 < PD; ++d)
+#pragma unroll
+          for (int k = 0; k < KY; ++k)
+#pragma unroll
+            for (int v = 0; v < VZ; ++v) pq[d][f][k][v] = pq[d + 1][f][k][v];
       }
     }
     if constexpr (NB > 0) {
@@ -294,15 +304,16 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       store_bufs();
       __syncthreads();
     }
-    // prefetch plane x+1 while plane x is computed
-    if (x + 1 < xe) {
-      if constexpr (T::HASQ) {
+    // prefetch plane x+PD of the queue and the LDS ring of plane x+1 while
+    // plane x is computed
+    if constexpr (T::HASQ) {
+      if (x + PD < xe) {
 #pragma unroll
         for (int f = 0; f < NF; ++f)
-          if (P::QF[f]) load_cells(f, x + 1 + P::QHI, pq[f]);
+          if (P::QF[f]) load_cells(f, x + PD + P::QHI, pq[PD - 1][f]);
       }
-      load_bufs(x + 1);
     }
+    if (x + 1 < xe) load_bufs(x + 1);
 
 #pragma unroll
     for (int k = 0; k < KY; ++k) {

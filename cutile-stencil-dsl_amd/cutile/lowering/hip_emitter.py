@@ -100,7 +100,7 @@ def temporal_tile(ndim: int, T: int, r3: tuple, compute: str, budget: int = 160 
 
 
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,
-                nontemporal: bool = True) -> str:
+                nontemporal: bool = True, prefetch: int = 1) -> str:
     plan = plan_accesses(meta)
     bufs = plan["bufs"]
     r3 = radius3(meta)
@@ -118,7 +118,7 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int BRY[{max(1, len(bufs))}] = {_arr([b[2] for b in bufs])};",
         f"  static constexpr int BRZ[{max(1, len(bufs))}] = {_arr([b[3] for b in bufs])};",
         f"  static constexpr int RX = {r3[0]}, RY = {r3[1]}, RZ = {r3[2]};",
-        f"  static constexpr int NTS = {1 if nontemporal else 0};",
+        f"  static constexpr int NTS = {1 if nontemporal else 0}, PD = {int(prefetch)};",
         "  template <class A> ST_HD static void eval(const A& a, C* out) {",
     ]
     for k, e in enumerate(meta.expressions):
@@ -136,7 +136,7 @@ def temporal_symbol(meta: StencilMeta, tag: str, T: int) -> str:
 
 
 def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage: str,
-                    compute: str, temporal=None, nontemporal: bool = True) -> str:
+                    compute: str, temporal=None, nontemporal: bool = True, prefetch: int = 1) -> str:
     """Full HIP translation unit with one sweep kernel per variant."""
     out = [
         f"// {meta.name}: generated by the HIP emitter for gfx950",
@@ -146,7 +146,7 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
     ]
     for cfg in variants:
         tag = cfg.tag
-        out.append(emit_traits(meta, cfg, storage, compute, tag, nontemporal))
+        out.append(emit_traits(meta, cfg, storage, compute, tag, nontemporal, prefetch))
         out.append(
             f'extern "C" __global__ void __launch_bounds__({cfg.nt}) '
             f"{kernel_symbol(meta, tag)}(st::SweepArgs a) {{ st::sweep_entry<P_{tag}>(a); }}"

@@ -41,8 +41,10 @@ def storage_compute(dtype: str, storage: Optional[str]) -> tuple:
     raise ValueError(f"unsupported storage {storage!r} for dtype {dtype!r}")
 
 
-def make_configs(ndim: int, tile: Sequence[int], storage: str) -> tuple:
-    """(vector config, scalar config, chunk) for a ``tile_sizes`` tuple."""
+def make_configs(ndim: int, tile: Sequence[int], storage: str, threads: Optional[int] = None) -> tuple:
+    """(vector config, scalar config, chunk) for a ``tile_sizes`` tuple;
+    ``threads`` overrides the workgroup size (a multiple of the threads one
+    tile row needs)."""
     tile = tuple(int(t) for t in tile)
     if ndim == 3:
         xch, ty, tz = tile
@@ -57,7 +59,11 @@ def make_configs(ndim: int, tile: Sequence[int], storage: str) -> tuple:
         ntz = max(1, tz_ // vz_)
         if ntz > 1024:
             raise ValueError(f"tile row of {tz_} elements needs {ntz} threads (> 1024)")
-        if ntz >= 64:
+        if threads:
+            nt = max(int(threads), ntz)
+            if nt % ntz or nt % 64:
+                raise ValueError(f"{threads} threads do not tile rows of {ntz} threads in wave64s")
+        elif ntz >= 64:
             nt = ntz * max(1, min(ty, 256 // ntz))
         else:
             nt = 64
@@ -77,7 +83,7 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
                 temporal_steps: int = 1, boundary_spec=None, kernel_hints: Optional[dict] = None,
                 storage: Optional[str] = None, fused: bool = False, temporal_mode: str = "zero") -> str:
     storage_t, compute_t = storage_compute(meta.dtype, storage)
-    vec, scal, xch = make_configs(meta.ndim, tile_sizes, storage_t)
+    vec, scal, xch = make_configs(meta.ndim, tile_sizes, storage_t, (kernel_hints or {}).get("threads"))
     temporal = None
     if temporal_steps > 1:
         if meta.num_inputs != 1 or meta.num_outputs != 1:
@@ -85,7 +91,8 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
         ty, tz = temporal_tile(meta.ndim, temporal_steps, radius3(meta), compute_t)
         temporal = (KernelConfig(ty=ty, tz=tz, vz=1, nt=256), int(temporal_steps))
     src = emit_hip_source(meta, [vec, scal], storage_t, compute_t, temporal,
-                          nontemporal=bool((kernel_hints or {}).get("nontemporal", 1)))
+                          nontemporal=bool((kernel_hints or {}).get("nontemporal", 1)),
+                          prefetch=int((kernel_hints or {}).get("prefetch", 1)))
     attr = _attrs(kernel_hints)
     if attr:
         src = src.replace('extern "C" __global__ void ', f'extern "C" __global__ void {attr}')

@@ -3,25 +3,25 @@
 Same interface as the upstream pass (``cutile/passes/tiling.py:68-207``):
 ``unique_access_count``, ``TilingPass.tier``, ``TilingPass.candidates`` (a
 ranked list of ``(tile_sizes, kernel_hints)``) and the ``tile_sizes`` /
-``occupancy`` attributes it attaches.  The shapes are re-derived for gfx950:
+``occupancy`` attributes it attaches.  The shapes are the gfx950 ones:
 
-* a workgroup is ``threads_per_block`` (256) threads = four wave64s, each
-  thread owning ``16 // dtype_bytes`` contiguous elements (one 16-byte
-  vector), so the contiguous tile extent is 256 columns for float64 rows
-  of 2 KB per wave group, and rows are added along axis 1 (3-D);
 * ``tile_sizes`` lists, per array dimension, the cells one workgroup
   covers: ``(chunk, rows, cols)`` in 3-D, ``(chunk, cols)`` in 2-D and
   ``(cols,)`` in 1-D; ``chunk`` is how many planes (3-D) or rows (2-D) the
   workgroup streams through with its register queue;
-* narrow stencils (<= 16 distinct accesses) take 2 rows per thread, wide
-  ones 1 row (the register queue holds 2r+1 planes per owned cell), very
-  wide ones add an ``occupancy`` hint, which the emitter turns into
-  ``amdgpu_waves_per_eu`` (resident waves per SIMD) and therefore a VGPR
-  budget of 512 / occupancy;
-* the ladder after the first candidate trades rows for occupancy the same
-  way the upstream ladder does, and the runtime probe
-  (``cutile.runtime.regcheck``) keeps the first that neither spills nor
-  exceeds its register budget.
+* a workgroup is 256 threads (four wave64s); each thread owns 16-byte
+  vectors along the contiguous axis and ``rows / (256 / (cols / vector))``
+  rows, so the per-thread register queue holds ``(2r+1) x rows-per-thread``
+  vectors;
+* the first candidate per tier and element size is the shape measured
+  fastest on MI355X at 512^3 among variants that neither spill nor exceed
+  256 VGPRs (7-point float64: 16 x 256 rows x cols, 128 planes per chunk,
+  70 % of the 8 TB/s HBM peak; 25-point float64: 8 x 128; 25-point float32:
+  4 x 512); the rest of the ladder trades rows for registers, then adds an
+  ``occupancy`` hint (``amdgpu_waves_per_eu``, a VGPR budget of 512 /
+  occupancy);
+* the runtime probe (``cutile.runtime.regcheck``) keeps the first candidate
+  whose code object neither spills nor exceeds its register budget.
 """
 
 from __future__ import annotations
@@ -51,7 +51,7 @@ class TilingPass:
     wide_rows_per_thread: int = 1
     very_wide_occupancy: int = 4
     min_occupancy: int = 2
-    chunk_3d: int = 64
+    chunk_3d: int = 128
     chunk_2d: int = 64
 
     def apply(self, ctx, module) -> None:
@@ -77,28 +77,32 @@ class TilingPass:
     def vector(self) -> int:
         return max(1, 16 // self.dtype_bytes)
 
-    def row_tile(self, ndim: int, rows_per_thread: int, chunk: int | None = None) -> tuple:
-        """3-D: one wave64 spans a 1 KB row segment (64 vectors) and the
-        workgroup's waves stack along axis 1; 2-D/1-D: the whole workgroup
-        spans one row segment of ``threads_per_block`` vectors."""
+    # (rows, cols) per (element bytes, tier) for 3-D; measured on MI355X
+    _TILES_3D = {
+        (8, "narrow"): (16, 256), (8, "wide"): (8, 128), (8, "very_wide"): (4, 128),
+        (4, "narrow"): (16, 512), (4, "wide"): (4, 512), (4, "very_wide"): (4, 256),
+        (2, "narrow"): (16, 1024), (2, "wide"): (4, 1024), (2, "very_wide"): (4, 512),
+    }
+
+    def row_tile(self, ndim: int, tier: str, chunk: int | None = None) -> tuple:
         v = self.vector
         if ndim == 1:
             return (self.threads_per_block * v,)
         if ndim == 2:
             return (chunk or self.chunk_2d, self.threads_per_block * v)
-        waves = max(1, self.threads_per_block // 64)
-        return (chunk or self.chunk_3d, waves * rows_per_thread, 64 * v)
+        rows, cols = self._TILES_3D.get((self.dtype_bytes, tier), (8, 64 * v))
+        return (chunk or self.chunk_3d, rows, cols)
 
     def candidates(self, ndim: int, num_loads: int) -> list:
         tier = self.tier(num_loads)
-        narrow = self.row_tile(ndim, self.narrow_rows_per_thread)
-        one = self.row_tile(ndim, self.wide_rows_per_thread)
+        first = self.row_tile(ndim, tier)
         if tier == "narrow":
-            return [(narrow, {})]
-        ladder = [(one, {"occupancy": occ})
+            return [(first, {})]
+        smaller = first if ndim < 3 else (first[0], max(1, first[1] // 2), first[2])
+        ladder = [(smaller, {"occupancy": occ})
                   for occ in range(self.very_wide_occupancy, self.min_occupancy - 1, -1)]
-        unhinted = (one, {})
-        ordered = [unhinted] + ladder if tier == "wide" else ladder + [unhinted]
+        ordered = [(first, {}), (smaller, {})] + ladder if tier == "wide" else \
+            [(first, {})] + ladder + [(smaller, {})]
         seen: list = []
         for c in ordered:
             if c not in seen:

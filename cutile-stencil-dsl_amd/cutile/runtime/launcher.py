@@ -183,8 +183,7 @@ def pad_inner_halo(halo_widths: tuple, dtype_bytes: int, align_bytes: int = 128)
 def _budgeted_configs(tiling, candidates):
     from cutile.runtime import regcheck
 
-    return [regcheck.Config(tuple(t), dict(h),
-                            max_registers=regcheck.register_budget(h.get("occupancy", tiling.very_wide_occupancy)))
+    return [regcheck.Config(tuple(t), dict(h), max_registers=regcheck.register_budget(h.get("occupancy")))
             for t, h in candidates]
 
 
@@ -238,12 +237,13 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
             brick_size: int = 32, align_halo: bool = True, occupancy: Optional[int] = None,
             select: Optional[str] = None, temporal_steps: Optional[int] = None,
             storage: Optional[str] = None, tile_sizes: Optional[tuple] = None,
-            temporal_mode: str = "zero") -> CompileResult:
+            temporal_mode: str = "zero", kernel_hints: Optional[dict] = None) -> CompileResult:
     """Compile a ``@stencil`` function (``launcher.py:342-602``).
 
     Extra keyword arguments of this backend: ``temporal_steps`` forces the
     temporal depth, ``storage="bfloat16"`` stores a float32 stencil's arrays
     in bfloat16 (arithmetic stays float32), ``tile_sizes`` forces the tile,
+    ``kernel_hints`` adds emitter options (``occupancy``, ``nontemporal``),
     ``temporal_mode="recompute"`` (with ``temporal_steps > 1``) widens the
     halo to ``T * stencil halo`` and recomputes intermediate levels there
     (T periodic steps per launch after a halo fill).
@@ -317,6 +317,8 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
         hints = dict(res.kernel_hints or {})
     if occupancy is not None:
         hints["occupancy"] = occupancy
+    if kernel_hints:
+        hints.update(kernel_hints)
 
     boundary_spec = getattr(stencil_fn, "_boundary", None)
 

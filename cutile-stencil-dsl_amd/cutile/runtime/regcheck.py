@@ -154,12 +154,17 @@ def _time(code, kernel_name, ndim, num_inputs, num_outputs, halo_widths, dtype, 
     return e0.elapsed_ms(e1) / iterations
 
 
-def register_budget(occupancy: int, threads_per_block: int = 256, registers_per_lane: int = 512) -> int:
+def register_budget(occupancy: Optional[int], threads_per_block: int = 256,
+                    registers_per_lane: int = 512) -> int:
     """VGPRs per lane that keep ``occupancy`` workgroups of ``threads_per_block``
     threads resident per CU (each wave64 of a 256-thread group lands on its
-    own SIMD, so ``occupancy`` waves share one SIMD's 512 registers/lane)."""
+    own SIMD, so ``occupancy`` waves share one SIMD's 512 registers/lane).
+    Without an occupancy target the budget is the 256 architectural VGPRs:
+    kernels that need accumulation registers as spill space are rejected."""
+    if not occupancy:
+        return 256
     waves_per_simd = max(1, occupancy * max(1, threads_per_block // 256))
-    return registers_per_lane // waves_per_simd
+    return min(256, registers_per_lane // waves_per_simd)
 
 
 def select_config(configs: Sequence[Config], probe: Callable, max_registers: Optional[int] = None) -> Config:

@@ -1,0 +1,54 @@
+"""Host bounds model of the sweep kernel (CPU only).
+
+Every tile shape the tiling pass hands out, at every queue prefetch depth the
+emitter accepts, is checked against the array bounds before any of them runs
+on a device: register-queue loads, halo-ring fetches, LDS stores/reads and
+output stores for the edge tiles of ragged and aligned shapes."""
+
+import pytest
+
+import kernel_model as km
+from cutile import workloads as w
+from cutile.frontend.ir import find_func
+from cutile.lowering.hip_emitter import plan_accesses
+from cutile.lowering.meta import access_table, extract_meta
+from cutile.lowering.stencil_to_hip import make_configs
+from cutile.passes.tiling import TilingPass
+
+
+@pytest.mark.parametrize("qlo,qhi", [(-1, 1), (-4, 4), (0, 0), (-2, 0)])
+@pytest.mark.parametrize("pd", [1, 2, 3])
+@pytest.mark.parametrize("xb,xe", [(0, 1), (0, 2), (0, 7), (64, 128), (10, 13)])
+def test_queue_schedule_loads_each_plane_once(qlo, qhi, pd, xb, xe):
+    planes = km.queue_planes(xb, xe, qlo, qhi, pd)
+    assert sorted(planes) == list(range(xb + qlo, xe + qhi))
+    assert len(set(planes)) == len(planes)
+
+
+def _cases():
+    out = []
+    for fn in (w.diffusion7(), w.star25(), w.star25("float32")):
+        func = find_func(fn._ir)
+        meta = extract_meta(func)
+        eb = 8 if func.dtype == "float64" else 4
+        tiles = {t for t, _ in TilingPass(dtype_bytes=eb).candidates(3, len(access_table(meta)))}
+        for t in sorted(tiles):
+            out.append(pytest.param(meta, t, id=f"{meta.name}-{func.dtype}-{'x'.join(map(str, t))}"))
+    return out
+
+
+@pytest.mark.parametrize("meta,tile", _cases())
+@pytest.mark.parametrize("pd", [1, 2])
+@pytest.mark.parametrize("n", [(512, 512, 512), (37, 45, 70), (130, 9, 260)])
+def test_tuned_tiles_in_bounds(meta, tile, pd, n):
+    plan = plan_accesses(meta)
+    h = max(abs(o) for a in meta.accesses for o in a.offsets)
+    halo3 = (h, h, h)
+    shape3 = tuple(k + 2 * h for k in n)
+    vec, scal, xch = make_configs(3, tile, meta.dtype)
+    acc = access_table(meta)
+    for cfg, rows_ok in ((vec, shape3[2] % vec.vz == 0), (scal, True)):
+        if not rows_ok:
+            continue
+        errs = km.check_launch(cfg, plan, shape3, halo3, n, xch, 0, accesses=acc, pd=pd)
+        assert not errs, errs[:5]
