@@ -38,15 +38,17 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel_prefix: str):
-    """HBM bytes per launch measured by rocprofv3 PMC (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration), as committed under profiles/."""
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per launch of ``kernel`` measured by rocprofv3 PMC
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the counter
+    reads for the device copy kernel of known size), as recorded in
+    profiles/traffic.txt (``<kernel symbol> <bytes per launch>`` lines)."""
     path = os.path.join(ROOT, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == kernel_prefix:
+        if len(parts) >= 2 and parts[0] == kernel:
             try:
                 return float(parts[1])
             except ValueError:
@@ -100,8 +102,11 @@ def measure(name, stencil_fn, interior_dev, n, steps, warmup, dtype_name, spec, 
     torch.cuda.synchronize()
     cells = n ** 3
     esz = a.element_size()
-    res = r.load_module()._KERNEL.resources(r.load_module()._KERNEL.pick([a], [b], halo))
+    K = r.load_module()._KERNEL
+    key = K.pick([a], [b], halo)
+    res = K.resources(key)
     out = {
+        "kernel": K.variants[key][0],
         "workload": name, "dtype": dtype_name, "interior": [n, n, n], "halo": list(halo),
         "tile": list(r.tile_sizes), "steps": steps, "ms_per_step": total_ms / steps,
         "kernel_ms": kern_ms, "gcell_updates_per_s": cells * steps / (total_ms * 1e-3) / 1e9,
@@ -140,7 +145,7 @@ def copy_ceiling(nbytes, iters=20):
     return {"bytes_moved": 2 * nbytes, "ms": ms, "gbs": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
-def cpu_baseline(n=256, steps=4):
+def cpu_baseline(n=256, steps=8):
     """The NumPy oracle (one core: NumPy ufuncs are single-threaded) on a
     256^3 sample of the headline workload."""
     sys.path.insert(0, ROOT)
@@ -221,7 +226,7 @@ def main():
         cpu = cpu_baseline()
         log(f"cpu baseline: {cpu['value']:.4f} Gcell-updates/s in {cpu['seconds']:.1f}s")
 
-    traffic = traffic_from_profiles("star25_sweep")
+    traffic = traffic_from_profiles(head["kernel"])
     line = {
         "metric": "Gcell-updates/s + %HBM roofline, batched FP32 vs plain FP64; serving p99 ms",
         "value": head["gcell_updates_per_s"],
