@@ -143,7 +143,7 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
     e.line(f"_VARIANTS = {json.dumps(variants)}")
     e.line(f"_KERNEL = StencilKernel({meta.name!r}, {nd}, {meta.num_inputs}, {meta.num_outputs}, "
            f"_HIP_SOURCE, {{k: tuple(v) for k, v in _VARIANTS.items()}}, STORAGE, {int(xch)}, "
-           f"queue_depth={qdepth})")
+           f"queue_depth={qdepth}, temporal_steps={int(temporal[1]) if temporal is not None else 1})")
     e.line(f"_HALO = ({', '.join(hvars)}{tr})")
 
     if boundary_spec is not None and meta.num_outputs == 1 and not fused:

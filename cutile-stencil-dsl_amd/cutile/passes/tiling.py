@@ -3,23 +3,25 @@
 Same interface as the upstream pass (``cutile/passes/tiling.py:68-207``):
 ``unique_access_count``, ``TilingPass.tier``, ``TilingPass.candidates`` (a
 ranked list of ``(tile_sizes, kernel_hints)``) and the ``tile_sizes`` /
-``occupancy`` attributes it attaches.  The shapes are the gfx950 ones:
+``occupancy`` (and ``threads``) attributes it attaches.  The shapes are the gfx950 ones:
 
 * ``tile_sizes`` lists, per array dimension, the cells one workgroup
   covers: ``(chunk, rows, cols)`` in 3-D, ``(chunk, cols)`` in 2-D and
-  ``(cols,)`` in 1-D; ``chunk`` is how many planes (3-D) or rows (2-D) the
-  workgroup streams through with its register queue;
-* a workgroup is 256 threads (four wave64s); each thread owns 16-byte
-  vectors along the contiguous axis and ``rows / (256 / (cols / vector))``
-  rows, so the per-thread register queue holds ``(2r+1) x rows-per-thread``
-  vectors;
+  ``(cols,)`` in 1-D; ``chunk`` is the longest run of planes (3-D) or rows
+  (2-D) one workgroup streams through with its register queue (the launcher
+  shortens it until the grid holds one workgroup per CU);
+* a workgroup is 256 threads (four wave64s) unless the candidate carries a
+  ``threads`` hint; each thread owns 16-byte vectors along the contiguous
+  axis and ``rows / (threads / (cols / vector))`` rows, so the per-thread
+  register queue holds ``(2r+1) x rows-per-thread`` vectors;
 * the first candidate per tier and element size is the shape measured
   fastest on MI355X at 512^3 among variants that neither spill nor exceed
-  256 VGPRs (7-point float64: 16 x 256 rows x cols, 128 planes per chunk,
-  70 % of the 8 TB/s HBM peak; 25-point float64: 8 x 128; 25-point float32:
-  4 x 512); the rest of the ladder trades rows for registers, then adds an
-  ``occupancy`` hint (``amdgpu_waves_per_eu``, a VGPR budget of 512 /
-  occupancy);
+  256 VGPRs (7-point float64: 16 x 256 rows x cols on 512 threads;
+  25-point float64: 16 x 128 on 512 threads; 25-point float32: 16 x 256 on
+  512 threads); taller tiles cut the halo rows re-read through L2, which is
+  what bounds the wide stencils.  The rest of the ladder halves the rows,
+  then adds an ``occupancy`` hint (``amdgpu_waves_per_eu``, a VGPR budget of
+  512 / occupancy);
 * the runtime probe (``cutile.runtime.regcheck``) keeps the first candidate
   whose code object neither spills nor exceeds its register budget.
 """
@@ -51,7 +53,7 @@ class TilingPass:
     wide_rows_per_thread: int = 1
     very_wide_occupancy: int = 4
     min_occupancy: int = 2
-    chunk_3d: int = 128
+    chunk_3d: int = 256
     chunk_2d: int = 64
 
     def apply(self, ctx, module) -> None:
@@ -62,6 +64,8 @@ class TilingPass:
                 op.attributes["tile_sizes"] = list(tile)
                 if "occupancy" in hints:
                     op.attributes["occupancy"] = hints["occupancy"]
+                if "threads" in hints:
+                    op.attributes["threads"] = hints["threads"]
 
     def tier(self, num_loads: int) -> str:
         if num_loads <= self.wide_stencil_loads:
@@ -77,32 +81,39 @@ class TilingPass:
     def vector(self) -> int:
         return max(1, 16 // self.dtype_bytes)
 
-    # (rows, cols) per (element bytes, tier) for 3-D; measured on MI355X
+    # (rows, cols, threads) per (element bytes, tier) for 3-D; the float64
+    # and float32 entries are the fastest non-spilling shapes measured on
+    # MI355X at 512^3, the bfloat16 ones keep the float32 register footprint
     _TILES_3D = {
-        (8, "narrow"): (16, 256), (8, "wide"): (8, 128), (8, "very_wide"): (4, 128),
-        (4, "narrow"): (16, 512), (4, "wide"): (4, 512), (4, "very_wide"): (4, 256),
-        (2, "narrow"): (16, 1024), (2, "wide"): (4, 1024), (2, "very_wide"): (4, 512),
+        (8, "narrow"): (16, 256, 512), (8, "wide"): (16, 128, 512), (8, "very_wide"): (4, 128, 256),
+        (4, "narrow"): (16, 512, 256), (4, "wide"): (16, 256, 512), (4, "very_wide"): (4, 256, 256),
+        (2, "narrow"): (16, 512, 256), (2, "wide"): (8, 512, 512), (2, "very_wide"): (4, 512, 256),
     }
 
     def row_tile(self, ndim: int, tier: str, chunk: int | None = None) -> tuple:
+        return self.shape(ndim, tier, chunk)[0]
+
+    def shape(self, ndim: int, tier: str, chunk: int | None = None) -> tuple:
+        """(tile_sizes, threads per workgroup) of the first candidate."""
         v = self.vector
         if ndim == 1:
-            return (self.threads_per_block * v,)
+            return (self.threads_per_block * v,), self.threads_per_block
         if ndim == 2:
-            return (chunk or self.chunk_2d, self.threads_per_block * v)
-        rows, cols = self._TILES_3D.get((self.dtype_bytes, tier), (8, 64 * v))
-        return (chunk or self.chunk_3d, rows, cols)
+            return (chunk or self.chunk_2d, self.threads_per_block * v), self.threads_per_block
+        rows, cols, nt = self._TILES_3D.get((self.dtype_bytes, tier), (8, 64 * v, 256))
+        return (chunk or self.chunk_3d, rows, cols), nt
 
     def candidates(self, ndim: int, num_loads: int) -> list:
         tier = self.tier(num_loads)
-        first = self.row_tile(ndim, tier)
+        first, nt = self.shape(ndim, tier)
+        base = {"threads": nt} if nt != self.threads_per_block else {}
         if tier == "narrow":
-            return [(first, {})]
+            return [(first, dict(base))]
         smaller = first if ndim < 3 else (first[0], max(1, first[1] // 2), first[2])
-        ladder = [(smaller, {"occupancy": occ})
+        ladder = [(smaller, dict(base, occupancy=occ))
                   for occ in range(self.very_wide_occupancy, self.min_occupancy - 1, -1)]
-        ordered = [(first, {}), (smaller, {})] + ladder if tier == "wide" else \
-            [(first, {})] + ladder + [(smaller, {})]
+        ordered = [(first, dict(base)), (smaller, dict(base))] + ladder if tier == "wide" else \
+            [(first, dict(base))] + ladder + [(smaller, dict(base))]
         seen: list = []
         for c in ordered:
             if c not in seen:

@@ -108,16 +108,19 @@ def cu_count() -> int:
     return _cu_cache[dev]
 
 
-def chunk_length(n0: int, tiles: int, default: int, queue_depth: int, waves_per_cu: int = 8) -> int:
-    """Planes per workgroup along the streaming axis: the compiled default,
-    shortened until the grid holds about ``waves_per_cu`` workgroups per CU,
-    but not below 4 planes per queue refill so the register-queue prologue
-    (``queue_depth - 1`` planes) stays a small fraction of the chunk."""
-    target = waves_per_cu * cu_count()
-    x = max(1, min(default, n0))
-    if tiles * -(-n0 // x) < target:
-        want = -(-n0 * tiles // target)
-        x = max(min(x, max(4 * max(1, queue_depth - 1), 4)), min(x, want))
+def chunk_length(n0: int, tiles: int, default: int, queue_depth: int) -> int:
+    """Planes per workgroup along the streaming axis.
+
+    The grid gets about one workgroup per CU: ``ceil(CUs / tiles)`` chunks of
+    ``ceil(n0 / chunks)`` planes, capped at the compiled chunk ``default``
+    and kept at 4 queue refills or more so the register-queue prologue
+    (``queue_depth - 1`` planes) stays a small fraction of the chunk.  On
+    MI355X at 512^3 one long-streaming workgroup per CU beat both shorter
+    chunks and several resident workgroups per CU for every sweep measured
+    (fewer prologue re-reads, neighbouring tiles advancing in step)."""
+    chunks = max(1, -(-cu_count() // max(1, tiles)))
+    x = min(max(1, default), -(-n0 // chunks))
+    x = max(x, 4 * max(1, queue_depth - 1))
     return max(1, min(x, n0))
 
 
@@ -130,7 +133,7 @@ class StencilKernel:
     """A compiled-on-demand family of sweep variants for one stencil."""
 
     def __init__(self, name: str, ndim: int, num_inputs: int, num_outputs: int, source: str,
-                 variants: dict, storage: str, xch: int, queue_depth: int = 3):
+                 variants: dict, storage: str, xch: int, queue_depth: int = 3, temporal_steps: int = 1):
         # variants: {"vec": (symbol, ty, tz, vz, nt), "scalar": (...)}
         self.name = name
         self.ndim = ndim
@@ -141,6 +144,7 @@ class StencilKernel:
         self.storage = storage
         self.xch = xch
         self.queue_depth = queue_depth
+        self.temporal_steps = temporal_steps
         self._modules: dict = {}
         self._fns: dict = {}
         self._lock = threading.Lock()
@@ -224,7 +228,9 @@ class StencilKernel:
         a.n0, a.n1, a.n2 = n
         a.ntz = -(-n[2] // tz)
         a.nty = -(-n[1] // ty)
-        x = xch if xch else chunk_length(n[0], a.ntz * a.nty, self.xch, self.queue_depth)
+        # a temporal launch warms up T levels: (queue_depth - 1) * T planes per chunk
+        depth = self.queue_depth if key != "temporal" else (self.queue_depth - 1) * self.temporal_steps + 1
+        x = xch if xch else chunk_length(n[0], a.ntz * a.nty, self.xch, depth)
         a.nch = -(-n[0] // x)
         a.xch = x
         a.nxcd = xcd_count(current_arch())

@@ -283,6 +283,8 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
             T = a.get("temporal_steps", 1)
             if "occupancy" in a:
                 hints["occupancy"] = a["occupancy"]
+            if "threads" in a and tile_sizes is None:
+                hints["threads"] = a["threads"]
             num_loads = unique_access_count(op)
             for k in ("flops", "unique_loads", "arithmetic_intensity", "bound"):
                 if k in a:

@@ -120,22 +120,22 @@ class TestPasses:
 
     def test_tiling_shapes_for_gfx950(self):
         t = TilingPass()
-        assert t.candidates(3, 7) == [((128, 16, 256), {})]
+        assert t.candidates(3, 7) == [((256, 16, 256), {"threads": 512})]
         assert t.candidates(2, 4) == [((64, 512), {})]
         assert t.candidates(1, 3) == [((512,),  {})]
-        assert TilingPass(dtype_bytes=4).candidates(3, 7)[0][0] == (128, 16, 512)
-        assert TilingPass(dtype_bytes=4).candidates(3, 25)[0][0] == (128, 4, 512)
+        assert TilingPass(dtype_bytes=4).candidates(3, 7) == [((256, 16, 512), {})]
+        assert TilingPass(dtype_bytes=4).candidates(3, 25)[0] == ((256, 16, 256), {"threads": 512})
         wide = t.candidates(3, 25)
-        assert wide[:2] == [((128, 8, 128), {}), ((128, 4, 128), {})]
-        assert wide[2][1] == {"occupancy": 4}
+        assert wide[:2] == [((256, 16, 128), {"threads": 512}), ((256, 8, 128), {"threads": 512})]
+        assert wide[2][1] == {"threads": 512, "occupancy": 4}
         very = t.candidates(3, 125)
-        assert very[0] == ((128, 4, 128), {}) and very[1][1] == {"occupancy": 4}
-        assert very[-1] == ((128, 2, 128), {})
+        assert very[0] == ((256, 4, 128), {}) and very[1][1] == {"occupancy": 4}
+        assert very[-1] == ((256, 2, 128), {})
         assert t.tier(16) == "narrow" and t.tier(17) == "wide" and t.tier(65) == "very_wide"
 
     def test_tiling_attribute_and_verify(self):
         a = _attrs(S(LAP3D), AnalysisPass(), TilingPass())
-        assert a["tile_sizes"] == [128, 16, 256]
+        assert a["tile_sizes"] == [256, 16, 256] and a["threads"] == 512
         m = S(LAP3D)._ir.clone()
         NormalizePass().apply(None, m)
         AnalysisPass().apply(None, m)
@@ -175,7 +175,7 @@ class TestCompileAPI:
         r = compile(S(LAP3D), temporal_blocking=False)
         assert r.name == "lap3d" and r.ndim == 3
         assert r.halo_widths == (1, 1, 16) and r.stencil_halo == (1, 1, 1)
-        assert r.tile_sizes == (128, 16, 256) and r.temporal_steps == 1
+        assert r.tile_sizes == (256, 16, 256) and r.temporal_steps == 1
         assert r.inputs == ("u",)
         assert "flops" in r.analysis and "bound" in r.analysis
         ast.parse(r.code)

@@ -31,24 +31,57 @@ def _cases():
         func = find_func(fn._ir)
         meta = extract_meta(func)
         eb = 8 if func.dtype == "float64" else 4
-        tiles = {t for t, _ in TilingPass(dtype_bytes=eb).candidates(3, len(access_table(meta)))}
-        for t in sorted(tiles):
-            out.append(pytest.param(meta, t, id=f"{meta.name}-{func.dtype}-{'x'.join(map(str, t))}"))
+        tiles = {(t, h.get("threads")) for t, h in TilingPass(dtype_bytes=eb).candidates(3, len(access_table(meta)))}
+        for t, nt in sorted(tiles, key=str):
+            out.append(pytest.param(meta, t, nt, id=f"{meta.name}-{func.dtype}-{'x'.join(map(str, t))}-n{nt}"))
     return out
 
 
-@pytest.mark.parametrize("meta,tile", _cases())
+@pytest.mark.parametrize("meta,tile,threads", _cases())
 @pytest.mark.parametrize("pd", [1, 2])
 @pytest.mark.parametrize("n", [(512, 512, 512), (37, 45, 70), (130, 9, 260)])
-def test_tuned_tiles_in_bounds(meta, tile, pd, n):
+def test_tuned_tiles_in_bounds(meta, tile, threads, pd, n):
     plan = plan_accesses(meta)
     h = max(abs(o) for a in meta.accesses for o in a.offsets)
     halo3 = (h, h, h)
     shape3 = tuple(k + 2 * h for k in n)
-    vec, scal, xch = make_configs(3, tile, meta.dtype)
+    vec, scal, xch = make_configs(3, tile, meta.dtype, threads=threads)
     acc = access_table(meta)
     for cfg, rows_ok in ((vec, shape3[2] % vec.vz == 0), (scal, True)):
         if not rows_ok:
             continue
-        errs = km.check_launch(cfg, plan, shape3, halo3, n, xch, 0, accesses=acc, pd=pd)
-        assert not errs, errs[:5]
+        for x in (xch, 64, 16):
+            errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=acc, pd=pd)
+            assert not errs, errs[:5]
+
+
+def _autotune_cases():
+    from cutile.runtime.autotune import generate_candidates
+
+    out = []
+    for fn in (w.diffusion7(), w.star25(), w.star25("float32")):
+        func = find_func(fn._ir)
+        meta = extract_meta(func)
+        eb = 8 if func.dtype == "float64" else 4
+        for tile, T, hints in generate_candidates(3, eb, len(access_table(meta))):
+            if T == 1:
+                out.append(pytest.param(meta, tile, hints.get("threads"),
+                                        id=f"{meta.name}-{func.dtype}-{'x'.join(map(str, tile))}-n{hints.get('threads')}"))
+    return out
+
+
+@pytest.mark.parametrize("meta,tile,threads", _autotune_cases())
+def test_autotune_candidates_in_bounds(meta, tile, threads):
+    plan = plan_accesses(meta)
+    h = max(abs(o) for a in meta.accesses for o in a.offsets)
+    acc = access_table(meta)
+    vec, scal, xch = make_configs(3, tile, meta.dtype, threads=threads)
+    for n in ((64, 64, 64), (37, 45, 70)):
+        halo3 = (h, h, -(-h // vec.vz) * vec.vz)
+        shape3 = tuple(k + 2 * hh for k, hh in zip(n, halo3))
+        for cfg in (vec, scal):
+            if shape3[2] % cfg.vz:
+                continue
+            for x in (8, 64):
+                errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=acc)
+                assert not errs, errs[:5]

@@ -1,0 +1,41 @@
+"""Register and scratch footprint of the default kernels (CPU: hiprtc + the
+code-object notes, no device).  A default variant that spills or needs more
+than the 256 architectural VGPRs is rejected before it can reach a GPU."""
+
+import pytest
+
+from cutile import compile, compile_fused
+from cutile import workloads as w
+from cutile.runtime import regcheck, rt
+from cutile.runtime.kernel import compile_options
+
+pytestmark = pytest.mark.skipif(not rt.available() or regcheck.readelf_path() is None,
+                                reason="needs libstencil_rt.so (hiprtc) and llvm-readelf")
+
+
+def _cases():
+    return [
+        ("heat2d", lambda: compile(w.heat2d())),
+        ("diffusion7_f64", lambda: compile(w.diffusion7())),
+        ("diffusion7_f32", lambda: compile(w.diffusion7("float32"))),
+        ("diffusion7_bf16", lambda: compile(w.diffusion7("float32"), storage="bfloat16")),
+        ("star25_f64", lambda: compile(w.star25())),
+        ("star25_f32", lambda: compile(w.star25("float32"))),
+        ("wave2d", lambda: compile(w.wave2d())),
+        ("gray_scott_f64", lambda: compile_fused(list(w.gray_scott()))),
+        ("gray_scott_f32", lambda: compile_fused(list(w.gray_scott("float32")))),
+        ("star25_f64_T2", lambda: compile(w.star25(), temporal_steps=2, temporal_mode="recompute")),
+        ("heat2d_T4", lambda: compile(w.heat2d(), temporal_steps=4, temporal_mode="recompute")),
+    ]
+
+
+@pytest.mark.parametrize("name,make", _cases(), ids=[c[0] for c in _cases()])
+def test_default_variants_fit_registers(name, make):
+    r = make()
+    res = regcheck.code_object_resources(rt.compile_only(r.hip_source, compile_options("gfx950")))
+    kernels = {k: v for k, v in res.items() if "_sweep_" in k or "_temporal_" in k}
+    assert kernels, res
+    for k, v in kernels.items():
+        assert v.stack_bytes == 0, f"{k} spills ({v.stack_bytes} B scratch)"
+        assert v.registers <= 256, f"{k} uses {v.registers} VGPRs"
+        assert v.lds_bytes <= 160 * 1024, f"{k} uses {v.lds_bytes} B LDS"
