@@ -26,7 +26,7 @@ from typing import Optional, Sequence
 
 from cutile.frontend.ir import find_func
 from cutile.lowering.emitter import CodeEmitter
-from cutile.lowering.hip_emitter import (ELEM_BYTES, KernelConfig, emit_hip_source, kernel_symbol,
+from cutile.lowering.hip_emitter import (ELEM_BYTES, KernelConfig, emit_hip_source, kernel_symbol, lds_bytes,
                                         plan_accesses, radius3, temporal_symbol, temporal_tile)
 from cutile.lowering.meta import StencilMeta, extract_fused_meta, extract_meta
 
@@ -72,6 +72,34 @@ def make_configs(ndim: int, tile: Sequence[int], storage: str, threads: Optional
     vec = cfg(tz, vz)
     scal = cfg(max(1, tz // vz), 1)
     return vec, scal, xch
+
+
+def fit_lds(meta: StencilMeta, tile: Sequence[int], hints: Optional[dict], storage: Optional[str],
+            budget: int = 160 * 1024) -> tuple:
+    """``(tile, hints)`` shrunk until the sweep's LDS plane buffers fit in
+    ``budget`` bytes (one CU's LDS): halve the rows, then the columns down to
+    64 vectors; a ``threads`` hint is dropped once the tile rows no longer
+    give every thread a row.  Stencils with many distinct planes and fields
+    need one buffer per (field, plane offset)."""
+    storage_t, compute = storage_compute(meta.dtype, storage)
+    tile = tuple(int(t) for t in tile)
+    hints = dict(hints or {})
+    nd = len(tile)
+    while True:
+        vec, _, _ = make_configs(nd, tile, storage_t, threads=hints.get("threads"))
+        if lds_bytes(meta, vec, compute) <= budget:
+            return tile, hints
+        if nd == 3 and tile[1] > 1:
+            tile = (tile[0], tile[1] // 2, tile[2])
+        elif tile[-1] > 64 * vec.vz:
+            tile = tile[:-1] + (tile[-1] // 2,)
+        else:
+            raise ValueError(f"{meta.name}: LDS plane buffers exceed {budget} bytes even at tile {tile}")
+        if "threads" in hints:
+            ntz = max(1, tile[-1] // vec.vz)
+            rows = tile[1] if nd == 3 else 1
+            if hints["threads"] // ntz > rows or hints["threads"] % ntz:
+                hints.pop("threads")
 
 
 def _attrs(kernel_hints: Optional[dict]) -> str:

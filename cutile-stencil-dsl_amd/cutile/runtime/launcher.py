@@ -303,8 +303,17 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
     if align_halo and layout != "bricked":
         halo = pad_inner_halo(halo, dtype_bytes)
 
+    from cutile.frontend.ir import find_func
+    from cutile.lowering.meta import extract_meta
+    from cutile.lowering.stencil_to_hip import fit_lds
+
+    meta = extract_meta(find_func(ir))
+
+    def _fit(t, h):
+        return fit_lds(meta, t, h, storage)
+
     tiling = next((p for p in pipeline.passes if isinstance(p, TilingPass)), TilingPass(dtype_bytes=dtype_bytes))
-    candidates = tiling.candidates(len(halo), num_loads)
+    candidates = [_fit(t, h) for t, h in tiling.candidates(len(halo), num_loads)]
     if tiles is None:
         tiles = tuple(candidates[0][0])
 
@@ -323,6 +332,7 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
         hints.update(kernel_hints)
 
     boundary_spec = getattr(stencil_fn, "_boundary", None)
+    tiles, hints = _fit(tiles, hints)
 
     probe_wanted = (len(candidates) > 1 and occupancy is None and tile_sizes is None
                     and not (autotune and domain is not None) and layout != "bricked")
@@ -364,7 +374,7 @@ def compile_fused(stencil_fns: list, domain=None, hw=None, tile_sizes=None, halo
     output per member, launcher ``launch_<name>(*inputs, *outputs)``."""
     from cutile.frontend.ir import ApplyOp
     from cutile.lowering.meta import extract_fused_meta
-    from cutile.lowering.stencil_to_hip import lower_fused_stencils_to_python
+    from cutile.lowering.stencil_to_hip import fit_lds, lower_fused_stencils_to_python
     from cutile.passes.tiling import TilingPass
     from cutile.runtime import regcheck
 
@@ -401,7 +411,8 @@ def compile_fused(stencil_fns: list, domain=None, hw=None, tile_sizes=None, halo
     tiling = TilingPass(dtype_bytes=dtype_bytes)
     candidates = [(tuple(tile_sizes), {})] if tile_sizes is not None else tiling.candidates(ndim, num_loads)
     if occupancy is not None:
-        candidates = [(candidates[0][0], {"occupancy": occupancy})]
+        candidates = [(candidates[0][0], dict(candidates[0][1], occupancy=occupancy))]
+    candidates = [fit_lds(meta, t, h, storage, lds) for t, h in candidates]
 
     def _lower(cfg):
         return lower_fused_stencils_to_python(modules, tile_sizes=cfg.tile_sizes, halo_widths=halo_widths,

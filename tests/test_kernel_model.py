@@ -85,3 +85,36 @@ def test_autotune_candidates_in_bounds(meta, tile, threads):
             for x in (8, 64):
                 errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=acc)
                 assert not errs, errs[:5]
+
+
+@pytest.mark.parametrize("seed", [2, 3, 10, 30, 41, 57, 73, 99])
+def test_random_stencil_launches_in_bounds(seed):
+    """The compiled tile of the random device-parity stencils (several fields,
+    several plane offsets, radius up to 4) at the parity test's shapes."""
+    from helpers import random_stencil
+
+    from cutile import compile
+
+    shapes = [(1, 1, 1), (1, 2, 2), (2, 1, 3), (2, 2, 2), (2, 4, 4), (3, 1, 2), (3, 2, 1), (3, 3, 3),
+              (3, 4, 2), (3, 1, 4)]
+    nd, nf, rad = shapes[seed % len(shapes)]
+    fn, _, _ = random_stencil(seed, nd, nf, rad, 12)
+    r = compile(fn, temporal_blocking=False)
+    meta = extract_meta(find_func(fn._ir))
+    plan = plan_accesses(meta)
+    acc = access_table(meta)
+    vec, scal, xch = make_configs(nd, r.tile_sizes, meta.dtype, threads=r.kernel_hints.get("threads"))
+    base = {1: (300,), 2: (37, 70), 3: (11, 13, 40)}[nd]
+    shape = tuple(b + 2 * h for b, h in zip(base, r.halo_widths))
+    if nd == 1:
+        s3, h3, n3 = (1, 1, shape[0]), (0, 0, r.halo_widths[0]), (1, 1, base[0])
+    elif nd == 2:
+        s3, h3, n3 = (shape[0], 1, shape[1]), (r.halo_widths[0], 0, r.halo_widths[1]), (base[0], 1, base[1])
+    else:
+        s3, h3, n3 = shape, tuple(r.halo_widths), base
+    for cfg in (vec, scal):
+        if s3[2] % cfg.vz or h3[2] % cfg.vz:
+            continue
+        for x in (xch, 4, 1):
+            errs = km.check_launch(cfg, plan, s3, h3, n3, x, 0, accesses=acc)
+            assert not errs, errs[:5]

@@ -39,3 +39,21 @@ def test_default_variants_fit_registers(name, make):
         assert v.stack_bytes == 0, f"{k} spills ({v.stack_bytes} B scratch)"
         assert v.registers <= 256, f"{k} uses {v.registers} VGPRs"
         assert v.lds_bytes <= 160 * 1024, f"{k} uses {v.lds_bytes} B LDS"
+
+
+_RANDOM_SHAPES = [(1, 1, 1), (1, 2, 2), (2, 1, 3), (2, 2, 2), (2, 4, 4), (3, 1, 2), (3, 2, 1), (3, 3, 3),
+                  (3, 4, 2), (3, 1, 4)]
+
+
+@pytest.mark.parametrize("seed", [2, 3, 10, 30, 41, 57, 73, 99])
+def test_random_stencils_build_within_lds(seed):
+    """Many fields x many plane offsets need one LDS buffer each: the tile
+    must shrink until they fit the 160 KiB of a CU, and the kernel build."""
+    from helpers import random_stencil
+
+    nd, nf, rad = _RANDOM_SHAPES[seed % len(_RANDOM_SHAPES)]
+    fn, _, _ = random_stencil(seed, nd, nf, rad, 12)
+    r = compile(fn, temporal_blocking=False)
+    res = regcheck.code_object_resources(rt.compile_only(r.hip_source, compile_options("gfx950")))
+    for k, v in res.items():
+        assert v.lds_bytes <= 160 * 1024, (k, v)
