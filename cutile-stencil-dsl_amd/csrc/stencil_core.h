@@ -88,6 +88,13 @@ ST_HD int xcd_swizzle(int i, int n, int nxcd) {
   return x * q + (x < r ? x : r) + j;
 }
 
+// compile-time integer sequences (hiprtc has no <utility>)
+template <int... I> struct iseq {};
+template <int N, int... I> struct mkseq : mkseq<N - 1, N - 1, I...> {};
+template <int... I> struct mkseq<0, I...> { using type = iseq<I...>; };
+template <int V> struct ic { static constexpr int value = V; };
+template <class F, int... I> ST_HD void for_each_ic(F&& f, iseq<I...>) { (f(ic<I>{}), ...); }
+
 template <class P> struct Tab {
   static constexpr int NTZ = P::TZ / P::VZ;
   static constexpr int NTY = P::NT / NTZ;
@@ -131,8 +138,9 @@ template <class P> struct Tab {
   }
 };
 
-// Per-cell view handed to the generated expression.
-template <class P> struct Cell {
+// Per-cell view handed to the generated expression.  The register queue is
+// a ring: at rotation R, slot (i + R) % QN holds plane x + QLO + i.
+template <class P, int R> struct Cell {
   using C = typename P::C;
   using T = Tab<P>;
   const C (*q)[T::QN][T::KY][P::VZ];
@@ -141,7 +149,15 @@ template <class P> struct Cell {
 
   template <int F, int DX, int DY, int DZ> ST_HD C g() const {
     if constexpr (DY == 0 && DZ == 0 && T::HASQ && DX >= P::QLO && DX <= P::QHI) {
-      return q[F][DX - P::QLO][k][v];
+      return q[F][(DX - P::QLO + R) % T::QN][k][v];
+    } else if constexpr (DY == 0 && DZ != 0 && DZ > -P::VZ && DZ < P::VZ && T::HASQ &&
+                         DX >= P::QLO && DX <= P::QHI && P::QF[F]) {
+      // a neighbour along z inside my own vector is in the queue as well (the
+      // LDS interior is copied from the same registers); v is a constant
+      // once the per-vector loop is unrolled
+      if (v + DZ >= 0 && v + DZ < P::VZ) return q[F][(DX - P::QLO + R) % T::QN][k][v + DZ];
+      constexpr int b = T::buf(F, DX);
+      return lds[T::off(b) + (ly + P::BRY[b]) * T::cols(b) + (lz + T::rzv(b) + DZ)];
     } else {
       constexpr int b = T::buf(F, DX);
       static_assert(b >= 0, "access has no LDS buffer");
@@ -230,7 +246,8 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     }
   };
 
-  auto store_bufs = [&]() {
+  auto store_bufs = [&](auto Rc) {
+    constexpr int R = decltype(Rc)::value;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       C* base = lds + T::off(b);
@@ -241,7 +258,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
             if (ty + k * NTY < TY) {
               C* d = base + (P::BRY[b] + ty + k * NTY) * T::cols(b) + T::rzv(b) + tz * VZ;
 #pragma unroll
-              for (int v = 0; v < VZ; ++v) d[v] = q[P::BF[b]][P::BDX[b] - P::QLO][k][v];
+              for (int v = 0; v < VZ; ++v) d[v] = q[P::BF[b]][(P::BDX[b] - P::QLO + R) % QN][k][v];
             }
           }
         }
@@ -267,7 +284,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     for (int f = 0; f < NF; ++f) {
       if (!P::QF[f]) continue;
 #pragma unroll
-      for (int i = 0; i + 1 < QN; ++i) load_cells(f, xb + P::QLO + i, q[f][i + 1]);
+      for (int i = 0; i + 1 < QN; ++i) load_cells(f, xb + P::QLO + i, q[f][i]);
 #pragma unroll
       for (int k = 0; k < PD; ++k)
         if (xb + k < xe) load_cells(f, xb + P::QHI + k, pq[k][f]);
@@ -275,22 +292,19 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   }
   load_bufs(xb);
 
-  for (int x = xb; x < xe; ++x) {
-    // install plane x+QHI at the back of the queue
+  // one plane: rotation R = (x - xb) % QN names the queue slots, so the
+  // queue never moves between registers
+  auto step = [&](auto Rc, int x) {
+    constexpr int R = decltype(Rc)::value;
+    // install plane x+QHI in the slot that held plane x-1+QLO
     if constexpr (T::HASQ) {
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         if (!P::QF[f]) continue;
 #pragma unroll
-        for (int i = 0; i + 1 < QN; ++i)
-#pragma unroll
-          for (int k = 0; k < KY; ++k)
-#pragma unroll
-            for (int v = 0; v < VZ; ++v) q[f][i][k][v] = q[f][i + 1][k][v];
-#pragma unroll
         for (int k = 0; k < KY; ++k)
 #pragma unroll
-          for (int v = 0; v < VZ; ++v) q[f][QN - 1][k][v] = pq[0][f][k][v];
+          for (int v = 0; v < VZ; ++v) q[f][(R + QN - 1) % QN][k][v] = pq[0][f][k][v];
 #pragma unroll
         for (int d = 0; d + 1 < PD; ++d)
 #pragma unroll
@@ -301,7 +315,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     }
     if constexpr (NB > 0) {
       __syncthreads();
-      store_bufs();
+      store_bufs(Rc);
       __syncthreads();
     }
     // prefetch plane x+PD of the queue and the LDS ring of plane x+1 while
@@ -323,7 +337,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       C res[NO][VZ];
 #pragma unroll
       for (int v = 0; v < VZ; ++v) {
-        Cell<P> c{q, lds, k, v, lyk, tz * VZ + v};
+        Cell<P, R> c{q, lds, k, v, lyk, tz * VZ + v};
         C o[NO];
         P::eval(c, o);
 #pragma unroll
@@ -353,6 +367,12 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
         }
       }
     }
+  };
+
+  for (int x = xb; x < xe; x += QN) {
+    for_each_ic([&](auto Rc) {
+      if (x + decltype(Rc)::value < xe) step(Rc, x + decltype(Rc)::value);
+    }, typename mkseq<QN>::type{});
   }
 }
 

@@ -86,8 +86,8 @@ class TilingPass:
     # MI355X at 512^3, the bfloat16 ones keep the float32 register footprint
     _TILES_3D = {
         (8, "narrow"): (16, 256, 512), (8, "wide"): (16, 128, 512), (8, "very_wide"): (4, 128, 256),
-        (4, "narrow"): (16, 512, 256), (4, "wide"): (16, 256, 512), (4, "very_wide"): (4, 256, 256),
-        (2, "narrow"): (16, 512, 256), (2, "wide"): (8, 512, 512), (2, "very_wide"): (4, 512, 256),
+        (4, "narrow"): (16, 256, 512), (4, "wide"): (16, 256, 512), (4, "very_wide"): (4, 256, 256),
+        (2, "narrow"): (16, 512, 512), (2, "wide"): (8, 512, 512), (2, "very_wide"): (4, 512, 256),
     }
 
     def row_tile(self, ndim: int, tier: str, chunk: int | None = None) -> tuple:

@@ -123,7 +123,7 @@ class TestPasses:
         assert t.candidates(3, 7) == [((256, 16, 256), {"threads": 512})]
         assert t.candidates(2, 4) == [((64, 512), {})]
         assert t.candidates(1, 3) == [((512,),  {})]
-        assert TilingPass(dtype_bytes=4).candidates(3, 7) == [((256, 16, 512), {})]
+        assert TilingPass(dtype_bytes=4).candidates(3, 7) == [((256, 16, 256), {"threads": 512})]
         assert TilingPass(dtype_bytes=4).candidates(3, 25)[0] == ((256, 16, 256), {"threads": 512})
         wide = t.candidates(3, 25)
         assert wide[:2] == [((256, 16, 128), {"threads": 512}), ((256, 8, 128), {"threads": 512})]
