@@ -155,3 +155,76 @@ def test_golden_vectors(oracle):
     c = 0.25 * (u[0, 1] + u[2, 1] + u[1, 0] + u[1, 2])
     one = oracle.time_march(u, w.heat2d()._fn, 2, (1, 1), 1, spec2d)
     assert one[1, 1] == c
+
+
+class TestMultiFieldPins:
+    """Pins of the multi-input / fused / two-level paths: closed forms on
+    uniform states and a scalar loop over every interior cell."""
+
+    def test_unread_parameter_is_zero(self, oracle):
+        f = _fn("def s(u, w, i):\n    return u[i] + 2.0 * w[i]\n")
+        u = np.arange(6, dtype=float)
+        got = oracle.eval_multi({"u": u}, f, 1, (1,))
+        np.testing.assert_array_equal(got, u[1:5])
+
+    def test_gray_scott_uniform_closed_form(self, oracle):
+        from cutile import workloads as w
+
+        gu, gv = w.gray_scott()
+        GS = w.GS
+        u0, v0 = 0.7, 0.2
+        u = np.full((6, 6), u0)
+        v = np.full((6, 6), v0)
+        res = oracle.time_march_multi({"u": u, "v": v}, [gu._fn, gv._fn], ["u", "v"], 2, (1, 1), 1,
+                                      w.periodic_spec(2))
+        # the Laplacian of a uniform field is 0: only the reaction terms act,
+        # both evaluated from the old state
+        eu = u0 + GS["dt"] * (GS["Du"] * (u0 + u0 + u0 + u0 - 4 * u0) - u0 * v0 * v0 + GS["F"] * (1.0 - u0))
+        ev = v0 + GS["dt"] * (GS["Dv"] * (v0 + v0 + v0 + v0 - 4 * v0) + u0 * v0 * v0 - (GS["F"] + GS["k"]) * v0)
+        assert np.all(res["u"] == eu) and np.all(res["v"] == ev)
+        # u = 1, v = 0 is a fixed point
+        res = oracle.time_march_multi({"u": np.ones((6, 6)), "v": np.zeros((6, 6))}, [gu._fn, gv._fn],
+                                      ["u", "v"], 2, (1, 1), 3, w.periodic_spec(2))
+        assert np.all(res["u"] == 1.0) and np.all(res["v"] == 0.0)
+
+    def test_gray_scott_step_scalar_loop(self, oracle):
+        from cutile import workloads as w
+
+        gu, gv = w.gray_scott()
+        GS = w.GS
+        rng = np.random.default_rng(5)
+        u = rng.random((7, 8))
+        v = rng.random((7, 8))
+        res = oracle.time_march_multi({"u": u, "v": v}, [gu._fn, gv._fn], ["u", "v"], 2, (1, 1), 1, None)
+        for i in range(1, 6):
+            for j in range(1, 7):
+                lu = u[i - 1, j] + u[i + 1, j] + u[i, j - 1] + u[i, j + 1] - 4 * u[i, j]
+                lv = v[i - 1, j] + v[i + 1, j] + v[i, j - 1] + v[i, j + 1] - 4 * v[i, j]
+                eu = u[i, j] + GS["dt"] * (GS["Du"] * lu - u[i, j] * v[i, j] * v[i, j] + GS["F"] * (1.0 - u[i, j]))
+                ev = v[i, j] + GS["dt"] * (GS["Dv"] * lv + u[i, j] * v[i, j] * v[i, j] - (GS["F"] + GS["k"]) * v[i, j])
+                assert res["u"][i, j] == eu and res["v"][i, j] == ev
+        # halos are the old values (no BC given): the device writes only interiors
+        assert res["u"][0, 3] == u[0, 3]
+
+    def test_leapfrog_closed_forms_and_scalar_loop(self, oracle):
+        from cutile import workloads as w
+
+        f = w.wave2d("float64")
+        # uniform levels under a periodic refill: L = 0, so u_new = 2u - uold
+        # (a straight line in time)
+        u, uold = np.full((5, 5), 3.0), np.full((5, 5), 1.0)
+        a, b = oracle.leapfrog(u, uold, f._fn, 2, (1, 1), 2, w.periodic_spec(2))
+        assert np.all(a[1:4, 1:4] == 7.0) and np.all(b[1:4, 1:4] == 5.0)
+        rng = np.random.default_rng(9)
+        u, uold = rng.random((6, 7)), rng.random((6, 7))
+        a, b = oracle.leapfrog(u, uold, f._fn, 2, (1, 1), 1, w.dirichlet_spec(2))
+        for i in range(1, 5):
+            for j in range(1, 6):
+                e = 2.0 * u[i, j] - uold[i, j] + 0.25 * (u[i - 1, j] + u[i + 1, j] + u[i, j - 1] + u[i, j + 1]
+                                                          - 4.0 * u[i, j])
+                assert a[i, j] == e
+        assert np.all(a[0] == 0.0) and np.array_equal(b, u)
+
+    def test_rel_max_err(self, oracle):
+        assert oracle.rel_max_err(np.array([1.0, 2.0]), np.array([1.0, 4.0])) == 0.5
+        assert oracle.rel_max_err(np.zeros(3), np.zeros(3)) == 0.0
