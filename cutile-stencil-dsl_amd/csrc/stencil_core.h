@@ -178,6 +178,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   using C = typename P::C;
   using T = Tab<P>;
   constexpr int NT = P::NT, TY = P::TY, TZ = P::TZ, VZ = P::VZ, NF = P::NF, NO = P::NO, NB = P::NB;
+  constexpr int NOUT = P::NOUT;  // arrays written: NO results (+ NO compensations)
   constexpr int NTZ = T::NTZ, NTY = T::NTY, KY = T::KY, QN = T::QN, NVB = T::max_nvb();
   static_assert(P::NT % NTZ == 0, "NT must be a multiple of TZ/VZ");
 
@@ -334,18 +335,35 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       const int lyk = ty + k * NTY;
       const int y = y0 + lyk;
       if (lyk >= TY || y >= a.n1 || !zin) continue;
-      C res[NO][VZ];
+      C res[NOUT][VZ];
 #pragma unroll
       for (int v = 0; v < VZ; ++v) {
         Cell<P, R> c{q, lds, k, v, lyk, tz * VZ + v};
         C o[NO];
         P::eval(c, o);
+        if constexpr (P::UPD == 0) {
 #pragma unroll
-        for (int m = 0; m < NO; ++m) res[m][v] = o[m];
+          for (int m = 0; m < NO; ++m) res[m][v] = o[m];
+        } else {
+          // time-update epilogue: output m is field TF[m] advanced by DT * rate
+          for_each_ic([&](auto Mc) {
+            constexpr int m = decltype(Mc)::value;
+            const C base = c.template g<P::TF[m], 0, 0, 0>();
+            if constexpr (P::UPD == 1) {
+              res[m][v] = base + P::DT * o[m];
+            } else {
+              // Kahan: carry what the storage rounding drops in CF[m]
+              const C y = P::DT * o[m] - c.template g<P::CF[m], 0, 0, 0>();
+              const C t = Cvt<S, C>::ld(Cvt<S, C>::st(base + y));
+              res[m][v] = t;
+              res[NO + m][v] = (t - base) - y;
+            }
+          }, typename mkseq<NO>::type{});
+        }
       }
       const i64 off = ((i64)(a.h0 + x) * s1 + (a.h1 + y)) * s2 + a.h2 + zc;
 #pragma unroll
-      for (int m = 0; m < NO; ++m) {
+      for (int m = 0; m < NOUT; ++m) {
         S* dst = reinterpret_cast<S*>(a.out[m]) + off;
         if (zc + VZ <= a.n2) {
           typename NtVec<S, VZ>::type w;

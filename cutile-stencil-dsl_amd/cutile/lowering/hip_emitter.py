@@ -108,7 +108,7 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"struct P_{tag} {{",
         f"  using S = {STORAGE_CTYPES[storage]};",
         f"  using C = {COMPUTE_CTYPES[compute]};",
-        f"  static constexpr int NF = {meta.num_inputs}, NO = {meta.num_outputs};",
+        f"  static constexpr int NF = {meta.num_inputs}, NO = {meta.num_exprs};",
         f"  static constexpr int NT = {cfg.nt}, TY = {cfg.ty}, TZ = {cfg.tz}, VZ = {cfg.vz};",
         f"  static constexpr int QLO = {plan['qlo']}, QHI = {plan['qhi']};",
         f"  static constexpr int QF[{max(1, meta.num_inputs)}] = {_arr(plan['qf'])};",
@@ -119,6 +119,21 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int BRZ[{max(1, len(bufs))}] = {_arr([b[3] for b in bufs])};",
         f"  static constexpr int RX = {r3[0]}, RY = {r3[1]}, RZ = {r3[2]};",
         f"  static constexpr int NTS = {1 if nontemporal else 0}, PD = {int(prefetch)};",
+    ]
+    upd = meta.update
+    ne = meta.num_exprs
+    if upd is None:
+        lines.append(f"  static constexpr int UPD = 0, NOUT = {meta.num_outputs};")
+        lines.append("  static constexpr int TF[1] = {0}, CF[1] = {0};")
+        lines.append("  static constexpr C DT = C(0);")
+    else:
+        mode = {"euler": 1, "kahan": 2}[upd["mode"]]
+        from cutile.lowering.meta import cpp_literal
+
+        lines.append(f"  static constexpr int UPD = {mode}, NOUT = {meta.num_outputs};")
+        lines.append(f"  static constexpr int TF[{ne}] = {_arr(upd['tf'])}, CF[{ne}] = {_arr(upd['cf'] or [0] * ne)};")
+        lines.append(f"  static constexpr C DT = {cpp_literal(upd['dt'], COMPUTE_CTYPES[compute])};")
+    lines += [
         "  template <class A> ST_HD static void eval(const A& a, C* out) {",
     ]
     for k, e in enumerate(meta.expressions):

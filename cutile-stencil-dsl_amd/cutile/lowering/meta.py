@@ -82,6 +82,7 @@ class StencilMeta:
     boundary: Optional[dict] = None
     flops: int = 0
     mapping: str = "rows"
+    update: Optional[dict] = None     # time-update epilogue, see apply_update
 
     @property
     def num_inputs(self) -> int:
@@ -89,6 +90,12 @@ class StencilMeta:
 
     @property
     def num_outputs(self) -> int:
+        """Arrays the kernel writes (rates' targets, plus compensations)."""
+        n = max(1, len(self.expressions))
+        return 2 * n if (self.update and self.update["mode"] == "kahan") else n
+
+    @property
+    def num_exprs(self) -> int:
         return max(1, len(self.expressions))
 
     @property
@@ -330,6 +337,51 @@ def extract_fused_meta(modules: Sequence, compute_dtype: Optional[str] = None,
     meta.mapping = mapping
     meta.output_names = names
     meta.stencil_inputs = [list(f.arg_names) for f in funcs]
+    return meta
+
+
+UPDATE_MODES = ("euler", "kahan")
+
+
+def apply_update(meta: StencilMeta, mode: str, dt: float, targets: Sequence[str]) -> StencilMeta:
+    """Turn rate stencils into a time update fused into the sweep epilogue.
+
+    Each expression k is read as the rate ``R_k`` of field ``targets[k]``;
+    the kernel stores ``u_k + dt * R_k`` (``"euler"``, in the compute type --
+    the same operations as a stencil written ``u[i, j] + dt * (...)``), or
+    the Kahan-compensated update (``"kahan"``): with the compensation array
+    ``c_<target>`` as an extra input and output,
+    ``y = dt*R - c;  t = round_storage(u + y);  c' = (t - u) - y;  u' = t``,
+    so the low-order part that the storage type drops is carried to the
+    next step (upstream's time steppers update in the array's precision,
+    ``rk_integrator.py:183-232``)."""
+    if mode not in UPDATE_MODES:
+        raise ValueError(f"update must be one of {UPDATE_MODES}, got {mode!r}")
+    targets = list(targets)
+    if len(targets) != meta.num_exprs:
+        raise ValueError(f"{meta.name}: {len(targets)} update targets for {meta.num_exprs} rate expression(s)")
+    zero = (0,) * meta.ndim
+    have = {(a.array_index, tuple(a.offsets)) for a in meta.accesses}
+
+    def centre(idx):
+        if (idx, zero) not in have:
+            meta.accesses.append(AccessInfo(idx, zero, _view_name(meta.input_names[idx], zero)))
+            have.add((idx, zero))
+
+    tf = []
+    for t in targets:
+        if t not in meta.input_names:
+            raise ValueError(f"{meta.name}: update target {t!r} is not an input of the rate stencil(s)")
+        tf.append(meta.input_names.index(t))
+        centre(tf[-1])
+    cf = []
+    if mode == "kahan":
+        for t in targets:
+            meta.input_names.append(f"c_{t}")
+            cf.append(len(meta.input_names) - 1)
+            centre(cf[-1])
+    meta.update = {"mode": mode, "dt": float(dt), "tf": tf, "cf": cf, "targets": targets}
+    meta.output_names = targets + ([f"c_{t}" for t in targets] if mode == "kahan" else [])
     return meta
 
 

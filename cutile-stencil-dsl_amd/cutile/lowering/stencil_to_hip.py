@@ -28,7 +28,7 @@ from cutile.frontend.ir import find_func
 from cutile.lowering.emitter import CodeEmitter
 from cutile.lowering.hip_emitter import (ELEM_BYTES, KernelConfig, emit_hip_source, kernel_symbol, lds_bytes,
                                         plan_accesses, radius3, temporal_symbol, temporal_tile)
-from cutile.lowering.meta import StencilMeta, extract_fused_meta, extract_meta
+from cutile.lowering.meta import StencilMeta, apply_update, extract_fused_meta, extract_meta
 
 
 def storage_compute(dtype: str, storage: Optional[str]) -> tuple:
@@ -139,7 +139,7 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
     tr = "," if nd == 1 else ""
     multi = meta.num_inputs > 1 or fused
     inputs = list(meta.input_names) if multi else ["u_in"]
-    outs = meta.output_params() if (fused or meta.num_outputs > 1) else ["u_out"]
+    outs = meta.output_params() if (fused or meta.num_outputs > 1 or meta.update) else ["u_out"]
 
     e = CodeEmitter()
     e.line(f'"""HIP kernel module for the {meta.name} stencil (generated for gfx950)."""')
@@ -211,7 +211,8 @@ def _spec_repr(spec) -> str:
 
 def lower_stencil_to_python(module, domain=None, tile_sizes=None, halo_widths=None,
                             temporal_steps: int = 1, boundary_spec=None, kernel_hints=None,
-                            spatial_term_order=None, storage=None, temporal_mode: str = "zero") -> str:
+                            spatial_term_order=None, storage=None, temporal_mode: str = "zero",
+                            update=None, dt: float = 1.0, targets=None) -> str:
     func = find_func(module)
     if tile_sizes is None:
         from cutile.passes.tiling import TilingPass, unique_access_count
@@ -222,6 +223,10 @@ def lower_stencil_to_python(module, domain=None, tile_sizes=None, halo_widths=No
         h = func.order // 2 if func.order else 1
         halo_widths = (h,) * func.ndim
     meta = extract_meta(func)
+    if update is not None:
+        if temporal_steps != 1:
+            raise NotImplementedError("a time-update epilogue runs one step per launch (temporal_steps=1)")
+        apply_update(meta, update, dt, targets or [meta.input_names[0]])
     if boundary_spec is None and meta.boundary is not None:
         boundary_spec = meta.boundary
     return emit_module(meta, tile_sizes, halo_widths, temporal_steps, boundary_spec, kernel_hints, storage,
@@ -230,11 +235,14 @@ def lower_stencil_to_python(module, domain=None, tile_sizes=None, halo_widths=No
 
 def lower_fused_stencils_to_python(modules: Sequence, tile_sizes=None, halo_widths=None,
                                    temporal_steps: int = 1, kernel_hints=None,
-                                   spatial_term_order=None, storage=None) -> str:
+                                   spatial_term_order=None, storage=None, update=None, dt: float = 1.0,
+                                   targets=None) -> str:
     if temporal_steps != 1:
         raise NotImplementedError(
             "temporal blocking of fused kernels is not supported (temporal_steps must be 1)")
     meta = extract_fused_meta(modules)
+    if update is not None:
+        apply_update(meta, update, dt, targets)
     if halo_widths is None:
         halo_widths = (max(1, meta.order // 2),) * meta.ndim
     if tile_sizes is None:

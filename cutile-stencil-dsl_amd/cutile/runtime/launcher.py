@@ -16,7 +16,7 @@ import math
 import os
 import tempfile
 from dataclasses import dataclass, field
-from typing import Any, Optional
+from typing import Any, Optional, Sequence
 
 from cutile.runtime.pipeline import Pipeline
 
@@ -237,7 +237,8 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
             brick_size: int = 32, align_halo: bool = True, occupancy: Optional[int] = None,
             select: Optional[str] = None, temporal_steps: Optional[int] = None,
             storage: Optional[str] = None, tile_sizes: Optional[tuple] = None,
-            temporal_mode: str = "zero", kernel_hints: Optional[dict] = None) -> CompileResult:
+            temporal_mode: str = "zero", kernel_hints: Optional[dict] = None,
+            update: Optional[str] = None, dt: float = 1.0, targets: Optional[Sequence[str]] = None) -> CompileResult:
     """Compile a ``@stencil`` function (``launcher.py:342-602``).
 
     Extra keyword arguments of this backend: ``temporal_steps`` forces the
@@ -247,6 +248,9 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
     ``temporal_mode="recompute"`` (with ``temporal_steps > 1``) widens the
     halo to ``T * stencil halo`` and recomputes intermediate levels there
     (T periodic steps per launch after a halo fill).
+    ``update="euler"|"kahan"`` (with ``dt`` and ``targets``) reads the
+    stencil as the rate of field ``targets[0]`` and fuses the time update
+    into the sweep (see ``cutile.lowering.meta.apply_update``).
     ``num_gpus > 1`` is rejected: this backend drives one GPU."""
     from cutile.frontend.ir import ApplyOp
     from cutile.lowering.stencil_to_hip import lower_stencil_to_python
@@ -296,6 +300,9 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
         T = int(temporal_steps)
     if T > 1 and len(_input_names(ir)) != 1:
         T = 1  # temporal chains need a single input (upstream breaks there too)
+    if update is not None:
+        T = 1
+        targets = list(targets) if targets else [_input_names(ir)[0]]
 
     stencil_halo = halo
     if T > 1 and temporal_mode == "recompute":
@@ -339,7 +346,8 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
     if probe_wanted:
         def _lower(cfg):
             return lower_stencil_to_python(ir.clone(), tile_sizes=cfg.tile_sizes, halo_widths=halo,
-                                           kernel_hints=cfg.kernel_hints, storage=storage)
+                                           kernel_hints=cfg.kernel_hints, storage=storage, update=update,
+                                           dt=dt, targets=targets)
 
         tiles, hints, resources = _choose_config(
             _budgeted_configs(tiling, candidates), _lower, kernel_name=f"{name}_kernel",
@@ -348,16 +356,23 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
 
     code = lower_stencil_to_python(ir.clone(), domain=domain, tile_sizes=tiles, halo_widths=halo,
                                    temporal_steps=T, boundary_spec=boundary_spec,
-                                   kernel_hints=hints, storage=storage, temporal_mode=temporal_mode)
+                                   kernel_hints=hints, storage=storage, temporal_mode=temporal_mode,
+                                   update=update, dt=dt, targets=targets)
     if layout == "bricked":
         from cutile.lowering.bricked import bricked_suffix
 
         code += bricked_suffix(name, brick_size)
     src = code.split("_HIP_SOURCE = r'''\n", 1)[1].split("'''", 1)[0] if "_HIP_SOURCE" in code else ""
+    inputs, outputs = _input_names(ir), ()
+    if update is not None:
+        from cutile.lowering.meta import apply_update
+
+        um = apply_update(extract_meta(find_func(ir)), update, dt, targets)
+        inputs, outputs = tuple(um.input_names), tuple(um.output_names)
     return CompileResult(
         name=name, code=code, ndim=stencil_fn._ndim or len(halo), halo_widths=tuple(halo),
         tile_sizes=tuple(tiles), temporal_steps=T, analysis=analysis, stencil_halo=tuple(stencil_halo),
-        kernel_hints=hints, resources=resources, inputs=_input_names(ir), dtype=dtype, storage=store,
+        kernel_hints=hints, resources=resources, inputs=inputs, outputs=outputs, dtype=dtype, storage=store,
         hip_source=src, temporal_mode=temporal_mode, _ref_fn=stencil_fn._fn, _ref_fns=[stencil_fn._fn],
     )
 
@@ -368,10 +383,14 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
 
 def compile_fused(stencil_fns: list, domain=None, hw=None, tile_sizes=None, halo_widths=None,
                   temporal_steps: int = 1, occupancy: Optional[int] = None, align_halo: bool = True,
-                  select: Optional[str] = None, storage: Optional[str] = None) -> CompileResult:
+                  select: Optional[str] = None, storage: Optional[str] = None, update: Optional[str] = None,
+                  dt: float = 1.0, targets: Optional[Sequence[str]] = None) -> CompileResult:
     """Several stencils over one domain in one kernel (``launcher.py:610-728``):
     inputs merged by parameter name, one load per distinct access, one
-    output per member, launcher ``launch_<name>(*inputs, *outputs)``."""
+    output per member, launcher ``launch_<name>(*inputs, *outputs)``.
+    With ``update`` the members are rates of fields ``targets`` and the
+    kernel stores the advanced fields (and, for ``"kahan"``, their
+    compensations ``c_<target>``, which are inputs and outputs as well)."""
     from cutile.frontend.ir import ApplyOp
     from cutile.lowering.meta import extract_fused_meta
     from cutile.lowering.stencil_to_hip import fit_lds, lower_fused_stencils_to_python
@@ -385,6 +404,10 @@ def compile_fused(stencil_fns: list, domain=None, hw=None, tile_sizes=None, halo
             "temporal blocking of fused kernels is not supported (temporal_steps must be 1)")
     modules = [fn._ir for fn in stencil_fns]
     meta = extract_fused_meta(modules)
+    if update is not None:
+        from cutile.lowering.meta import apply_update
+
+        apply_update(meta, update, dt, targets)
     ndim = meta.ndim
     dtype = getattr(stencil_fns[0], "_dtype", "float64") or "float64"
     store = storage or dtype
@@ -416,7 +439,8 @@ def compile_fused(stencil_fns: list, domain=None, hw=None, tile_sizes=None, halo
 
     def _lower(cfg):
         return lower_fused_stencils_to_python(modules, tile_sizes=cfg.tile_sizes, halo_widths=halo_widths,
-                                              kernel_hints=cfg.kernel_hints, storage=storage)
+                                              kernel_hints=cfg.kernel_hints, storage=storage, update=update,
+                                              dt=dt, targets=targets)
 
     resources = None
     if len(candidates) > 1:

@@ -160,12 +160,17 @@ class TimeStepper:
 class FusedStepper:
     """Advance several coupled fields with a fused kernel (``compile_fused``):
     every member reads the old state; member k's output becomes field
-    ``targets[k]``; each updated field gets its halo refilled."""
+    ``targets[k]``; each field in ``fill`` (default: all targets) gets its
+    halo refilled.  A time-update result (``update="euler"|"kahan"``) writes
+    its fields and compensations in ``result.outputs`` order, so
+    ``targets=result.outputs`` and ``fill`` = the fields whose halo the
+    stencil reads (compensations are read at the centre only)."""
 
-    def __init__(self, result, boundary, targets: Sequence[str]):
+    def __init__(self, result, boundary, targets: Sequence[str], fill: Optional[Sequence[str]] = None):
         self.r = result
         self.spec = boundary
         self.targets = list(targets)
+        self.fill_names = list(fill) if fill is not None else list(targets)
         self.halo = result.halo_widths
         self._k = result.load_module()._KERNEL
 
@@ -178,7 +183,7 @@ class FusedStepper:
             ins = [fields[n] for n in self.r.inputs]
             outs = [scratch[t] for t in self.targets]
             self._k.launch(ins, outs, self.halo, stream, validate=False)
-            for t in self.targets:
+            for t in self.fill_names:
                 _boundary.fill(scratch[t], self.spec, self.halo, stream)
             for t in self.targets:
                 fields[t], scratch[t] = scratch[t], fields[t]

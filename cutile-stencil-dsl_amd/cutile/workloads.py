@@ -79,6 +79,25 @@ def gs_v(u, v, i, j):
     return u, v
 
 
+def gray_scott_rates(dtype: str = "float64"):
+    """The Gray-Scott right-hand sides R_u, R_v (the same operator trees as
+    the parenthesised parts of ``gray_scott``): compiled with
+    ``compile_fused(..., update="euler"|"kahan", dt=GS["dt"], targets=["u", "v"])``
+    the kernel stores ``u + dt*R_u`` and ``v + dt*R_v``."""
+    env = dict(GS)
+    ru = stencil_from_source("""
+def gs_ru(u, v, i, j):
+    return (Du * (u[i - 1, j] + u[i + 1, j] + u[i, j - 1] + u[i, j + 1] - 4 * u[i, j])
+            - u[i, j] * v[i, j] * v[i, j] + F * (1.0 - u[i, j]))
+""", env=env, dtype=dtype)
+    rv = stencil_from_source("""
+def gs_rv(u, v, i, j):
+    return (Dv * (v[i - 1, j] + v[i + 1, j] + v[i, j - 1] + v[i, j + 1] - 4 * v[i, j])
+            + u[i, j] * v[i, j] * v[i, j] - (F + k) * v[i, j])
+""", env=env, dtype=dtype)
+    return ru, rv
+
+
 def wave2d(dtype: str = "float32", c2: float = 0.25):
     return stencil_from_source(f"""
 def wave2d(u, uold, i, j):

@@ -264,6 +264,49 @@ def leapfrog(u: np.ndarray, uold: np.ndarray, fn: Callable, ndim: int, halo: Seq
     return a, b
 
 
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """float32 values rounded to the nearest bfloat16 (ties to even, NaN kept
+    quiet), returned as float32 -- the rounding of the device's bf16 stores."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    u = x.view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+    r = np.where(nan, ((u >> 16) | 0x40) & 0xFFFF, r)
+    return (r.astype(np.uint32) << 16).view(np.float32)
+
+
+def to_storage(x: np.ndarray, storage: str) -> np.ndarray:
+    return bf16_round(x) if storage == "bfloat16" else x
+
+
+def time_update(fields: dict, rate_fns: Sequence[Callable], targets: Sequence[str], ndim: int,
+                halo: Sequence[int], dt: float, mode: str = "euler", comps: dict | None = None,
+                storage: str | None = None) -> None:
+    """One step of ``u_k <- u_k + dt * R_k`` in place on the interiors.
+
+    ``R_k`` are the rate functions evaluated on the old fields (the compute
+    type is the arrays' dtype; bfloat16 storage is held as float32 arrays of
+    bf16 values).  ``"euler"`` stores ``round(u + dt*R)``; ``"kahan"`` keeps a
+    compensation per field: ``y = dt*R - c; t = round(u + y);
+    c = round((t - u) - y); u = t``."""
+    shape = next(iter(fields.values())).shape
+    dtype = next(iter(fields.values())).dtype
+    I = interior(shape, halo)
+    dtc = dtype.type(dt)
+    rates = [eval_multi(fields, fn, ndim, halo) for fn in rate_fns]
+    for t, R in zip(targets, rates):
+        base = fields[t][I].copy()
+        if mode == "euler":
+            fields[t][I] = to_storage(base + dtc * R, storage)
+        elif mode == "kahan":
+            y = dtc * R - comps[t][I]
+            tn = to_storage(base + y, storage)
+            comps[t][I] = to_storage((tn - base) - y, storage)
+            fields[t][I] = tn
+        else:
+            raise ValueError(mode)
+
+
 def rel_max_err(x: np.ndarray, ref: np.ndarray) -> float:
     """||x - ref||_inf / ||ref||_inf."""
     x = np.asarray(x, dtype=np.float64)

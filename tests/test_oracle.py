@@ -228,3 +228,43 @@ class TestMultiFieldPins:
     def test_rel_max_err(self, oracle):
         assert oracle.rel_max_err(np.array([1.0, 2.0]), np.array([1.0, 4.0])) == 0.5
         assert oracle.rel_max_err(np.zeros(3), np.zeros(3)) == 0.0
+
+
+class TestTimeUpdatePins:
+    def test_bf16_round_matches_torch(self, oracle):
+        import torch
+
+        rng = np.random.default_rng(1)
+        x = np.concatenate([rng.standard_normal(5000).astype(np.float32) * 10.0 ** rng.integers(-30, 30, 5000),
+                            np.float32([0.0, -0.0, 1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, np.inf, -np.inf, 3e38])])
+        x = x.astype(np.float32)
+        ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+        np.testing.assert_array_equal(oracle.bf16_round(x), ref)
+        assert np.isnan(oracle.bf16_round(np.float32([np.nan]))).all()
+
+    def test_euler_update_equals_the_full_form_stencil(self, oracle):
+        from cutile import workloads as w
+
+        rng = np.random.default_rng(3)
+        u, v = rng.random((10, 12)), rng.random((10, 12))
+        full = oracle.time_march_multi({"u": u, "v": v}, [f._fn for f in w.gray_scott()], ["u", "v"], 2,
+                                       (1, 1), 3, w.periodic_spec(2))
+        f = {"u": u.copy(), "v": v.copy()}
+        for _ in range(3):
+            oracle.time_update(f, [r._fn for r in w.gray_scott_rates()], ["u", "v"], 2, (1, 1), w.GS["dt"])
+            for n in ("u", "v"):
+                oracle.apply_boundary(f[n], w.periodic_spec(2), (1, 1))
+        assert np.array_equal(f["u"], full["u"]) and np.array_equal(f["v"], full["v"])
+
+    def test_kahan_carries_what_bf16_drops(self, oracle):
+        # a constant rate of 2^-10 on u = 1: bf16 (8-bit mantissa) drops every
+        # increment, the compensated sum keeps u - c equal to the exact sum
+        f = _fn("def r(u, i):\n    return u[i] * 0.0 + 0.0009765625\n")
+        u = np.ones(6, dtype=np.float32)
+        c = np.zeros(6, dtype=np.float32)
+        plain = np.ones(6, dtype=np.float32)
+        for n in range(1, 40):
+            oracle.time_update({"u": u}, [f], ["u"], 1, (1,), 1.0, "kahan", {"u": c}, "bfloat16")
+            oracle.time_update({"u": plain}, [f], ["u"], 1, (1,), 1.0, "euler", None, "bfloat16")
+            assert np.all(u[1:5] - c[1:5] == np.float32(1.0 + n * 2 ** -10))
+            assert np.all(plain[1:5] == 1.0)

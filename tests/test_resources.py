@@ -26,6 +26,10 @@ def _cases():
         ("gray_scott_f32", lambda: compile_fused(list(w.gray_scott("float32")))),
         ("star25_f64_T2", lambda: compile(w.star25(), temporal_steps=2, temporal_mode="recompute")),
         ("heat2d_T4", lambda: compile(w.heat2d(), temporal_steps=4, temporal_mode="recompute")),
+        ("gray_scott_kahan_bf16", lambda: compile_fused(list(w.gray_scott_rates("float32")), update="kahan",
+                                                        targets=["u", "v"], storage="bfloat16")),
+        ("gray_scott_kahan_f32", lambda: compile_fused(list(w.gray_scott_rates("float32")), update="kahan",
+                                                       targets=["u", "v"])),
     ]
 
 
