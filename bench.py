@@ -145,6 +145,260 @@ def copy_ceiling(nbytes, iters=20):
     return {"bytes_moved": 2 * nbytes, "ms": ms, "gbs": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
+def _timed(stream, fn, rt):
+    e0, e1 = rt.Event(), rt.Event()
+    e0.record(stream)
+    fn()
+    e1.record(stream)
+    return e0.elapsed_ms(e1)
+
+
+def temporal_depths(interior_dev, n, steps=48):
+    """25-point star, float64 and float32, temporal depth 1, 2 and 4 (one
+    launch advances T steps; periodic halo T*4 deep refilled per launch),
+    ``steps`` steps as one captured hipGraph."""
+    import torch
+
+    from cutile.runtime import rt
+    from cutile.runtime.timestep import TimeStepper
+    from cutile.workloads import periodic_spec, star25
+
+    out = {}
+    for dt_name in ("float64", "float32"):
+        for T in (1, 2, 4):
+            key = f"star25_{dt_name}_T{T}"
+            st = TimeStepper(star25(dt_name), periodic_spec(3), temporal_steps=T)
+            u = st.empty_like_interior((n, n, n))
+            u[tuple(slice(h, h + n) for h in st.halo)] = interior_dev.to(u.dtype)
+            # capture the step loop once (JIT build, one replay), then time replays
+            st.march(u, steps, graph=True, graph_steps=steps)
+            torch.cuda.synchronize()
+            g = st._graph
+            ms = _timed(g.stream, g.launch, rt)
+            torch.cuda.synchronize()
+            out[key] = {"T": T, "dtype": dt_name, "steps": steps, "halo": list(st.halo),
+                        "ms_per_step": ms / steps, "gcell_updates_per_s": n ** 3 * steps / (ms * 1e-3) / 1e9,
+                        "alg_bytes_per_update": 2 * u.element_size() / T}
+            log(f"{key}: {ms / steps:.4f} ms/step, {out[key]['gcell_updates_per_s']:.1f} Gcell-updates/s")
+            del u, st, g
+            torch.cuda.empty_cache()
+    return out
+
+
+def heat2d_c1(steps=100, n=1024):
+    """2-D heat 5-point, float64, 1024^2, zero Dirichlet, 100 steps (graph);
+    the NumPy oracle on the same run for comparison."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, ROOT)
+    import oracle
+    from cutile.runtime import rt
+    from cutile.runtime.timestep import TimeStepper
+    from cutile.workloads import dirichlet_spec, heat2d, heat2d_init
+
+    st = TimeStepper(heat2d(), dirichlet_spec(2))
+    u0 = heat2d_init(n, st.halo, seed=0)
+    u = torch.from_numpy(u0).cuda()
+    res = st.march(u, steps, graph=True, graph_steps=steps)     # capture + one replay
+    torch.cuda.synchronize()
+    got = res.cpu().numpy()
+    g = st._graph
+    ms = _timed(g.stream, g.launch, rt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ref = oracle.time_march(u0, heat2d()._fn, 2, st.halo, steps, dirichlet_spec(2))
+    cpu_s = time.perf_counter() - t0
+    I = oracle.interior(u0.shape, st.halo)
+    exact = bool(np.array_equal(got[I], ref[I]))
+    out = {"ms_per_step": ms / steps, "gcell_updates_per_s": n * n * steps / (ms * 1e-3) / 1e9,
+           "cpu_numpy_gcell_updates_per_s": n * n * steps / cpu_s / 1e9, "bit_exact_vs_numpy": exact}
+    log(f"heat2d 1024^2 x{steps}: {out['gcell_updates_per_s']:.1f} Gcell-updates/s (NumPy "
+        f"{out['cpu_numpy_gcell_updates_per_s']:.3f}), bit-exact={exact}")
+    return out
+
+
+def gray_scott_c3(n=4096, steps=1000):
+    """Fused Gray-Scott 4096^2, periodic, 1000 steps: float64 (reference run),
+    float32, bfloat16 storage with a Kahan-compensated update (float32
+    arithmetic); error of each against the float64 run after all steps."""
+    import torch
+
+    from cutile import compile_fused
+    from cutile.runtime import rt
+    from cutile.runtime.kernel import default_stream
+    from cutile.runtime.timestep import FusedStepper
+    from cutile.workloads import GS, gray_scott, gray_scott_init, gray_scott_rates, periodic_spec
+
+    spec = periodic_spec(2)
+    variants = [
+        ("float64", compile_fused(list(gray_scott("float64"))), None),
+        ("float32", compile_fused(list(gray_scott("float32"))), None),
+        ("bfloat16_kahan", compile_fused(list(gray_scott_rates("float32")), update="kahan", dt=GS["dt"],
+                                         targets=["u", "v"], storage="bfloat16"), "kahan"),
+        ("float32_kahan", compile_fused(list(gray_scott_rates("float32")), update="kahan", dt=GS["dt"],
+                                        targets=["u", "v"]), "kahan"),
+    ]
+    out, finals = {}, {}
+    for name, r, upd in variants:
+        u0, v0 = gray_scott_init(n, r.halo_widths, seed=0)
+        tdt = getattr(torch, r.storage)
+        f = {"u": torch.from_numpy(u0).cuda().to(tdt), "v": torch.from_numpy(v0).cuda().to(tdt)}
+        if upd:
+            f["c_u"], f["c_v"] = torch.zeros_like(f["u"]), torch.zeros_like(f["v"])
+        targets = list(r.outputs) if upd else ["u", "v"]
+        scratch = {t: torch.empty_like(f[t]) for t in targets}
+        stp = FusedStepper(r, spec, targets, fill=["u", "v"])
+        for t in ("u", "v"):
+            from cutile.runtime import boundary
+
+            boundary.fill(f[t], spec, r.halo_widths)
+        stp.enqueue(f, scratch, 2)               # JIT + warm-up (even count: buffers come back)
+        torch.cuda.synchronize()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            g = rt.Graph(side.cuda_stream)
+            with g:
+                stp.enqueue(f, scratch, 2, side.cuda_stream)
+            ms = _timed(side.cuda_stream, lambda: [g.launch() for _ in range(steps // 2)], rt)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        ms_total = ms
+        # the warm-up advanced 2 steps: the state compared is after steps + 2
+        h = r.halo_widths
+        finals[name] = {t: f[t][h[0]:h[0] + n, h[1]:h[1] + n].to(torch.float64) for t in ("u", "v")}
+        esz = f["u"].element_size()
+        nbytes = (2 + 2 + (4 if upd else 0)) * esz
+        out[name] = {"ms_per_step": ms_total / steps, "gcell_updates_per_s": n * n * steps / (ms_total * 1e-3) / 1e9,
+                     "alg_bytes_per_update": nbytes,
+                     "kernel_gbs": n * n * nbytes * steps / (ms_total * 1e-3) / 1e9}
+        log(f"gray-scott {name}: {ms_total / steps * 1e3:.1f} us/step, {out[name]['gcell_updates_per_s']:.1f} "
+            f"Gcell-updates/s")
+        del f, scratch, g, stp
+        torch.cuda.empty_cache()
+    ref = finals["float64"]
+    for name in out:
+        if name == "float64":
+            continue
+        err = max(float((finals[name][t] - ref[t]).abs().max() / ref[t].abs().max()) for t in ("u", "v"))
+        out[name]["rel_err_vs_float64"] = err
+        log(f"gray-scott {name}: rel max err vs float64 after {steps + 2} steps = {err:.3e}")
+    return out
+
+
+def batched_wave_c4(count=512, steps=200, seed=0):
+    """512 leapfrog wave instances (sides log-uniform in [64, 2048], Gaussian
+    pulse, zero Dirichlet): the batched float32 persistent kernel (graph of a
+    full buffer rotation) against plain float64 one-launch-per-instance
+    (also graph-captured), same step count; error of float32 vs float64."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from cutile import compile
+    from cutile.runtime import rt
+    from cutile.runtime.batched import BatchedStencil, LeapfrogBatch
+    from cutile.runtime.kernel import default_stream
+    from cutile.workloads import wave2d, wave_sides
+
+    sides = wave_sides(count, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    centres = rng.uniform(0, 1, size=(count, 2))
+    cells = sum(s * s for s in sides)
+
+    def pulse(s, c, dtype, halo, cols):
+        x = torch.arange(s, dtype=torch.float64, device="cuda")
+        sig = 0.05 * s
+        g = torch.exp(-(((x - c[0] * s) ** 2)[:, None] + ((x - c[1] * s) ** 2)[None, :]) / (2 * sig * sig))
+        a = torch.zeros((s + 2 * halo[0], cols + 2 * halo[1]), dtype=dtype, device="cuda")
+        a[halo[0]:halo[0] + s, halo[1]:halo[1] + s] = g.to(dtype)
+        return a
+
+    # --- batched float32 --------------------------------------------------
+    bs = BatchedStencil(wave2d("float32"))
+    h32 = (1, bs.vz)
+    pairs, inter = [], []
+    for s, c in zip(sides, centres):
+        cols = -(-s // bs.vz) * bs.vz
+        u = pulse(s, c, torch.float32, h32, cols)
+        pairs.append((u, u.clone()))
+        inter.append((s, s))
+    lf = LeapfrogBatch(bs, pairs, h32, inter)
+    lf.step()
+    lf.step()
+    lf.step()                                      # JIT + one rotation (back to buffer order 0)
+    torch.cuda.synchronize()
+    s = default_stream()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        g = rt.Graph(side.cuda_stream)
+        with g:
+            for _ in range(3):
+                lf.step(side.cuda_stream)
+        reps, rest = divmod(steps, 3)
+        def run32():
+            for _ in range(reps):
+                g.launch()
+            for _ in range(rest):
+                lf.step(side.cuda_stream)
+        ms32 = _timed(side.cuda_stream, run32, rt)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    # --- plain float64, one launch per instance ---------------------------------
+    r64 = compile(wave2d("float64"))
+    k64 = r64.load_module()._KERNEL
+    h64 = r64.halo_widths
+    bufs = []
+    for sd, c in zip(sides, centres):
+        u = pulse(sd, c, torch.float64, h64, sd)
+        bufs.append([u, u.clone(), torch.zeros_like(u)])
+    rots = [(0, 1, 2), (2, 0, 1), (1, 2, 0)]
+
+    def step64(k, stream):
+        i, j, o = rots[k % 3]
+        for b in bufs:
+            k64.launch([b[i], b[j]], [b[o]], h64, stream, validate=False)
+
+    for k in range(3):
+        step64(k, s)
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        g64 = rt.Graph(side.cuda_stream)
+        with g64:
+            for k in range(3):
+                step64(k, side.cuda_stream)
+        def run64():
+            for _ in range(reps):
+                g64.launch()
+            for k in range(rest):
+                step64(k, side.cuda_stream)
+        ms64 = _timed(side.cuda_stream, run64, rt)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    # both did 3 + steps steps; compare u of every instance
+    err = 0.0
+    for idx, (sd, b) in enumerate(zip(sides, bufs)):
+        u32 = lf.current(idx)[0][1:1 + sd, h32[1]:h32[1] + sd].to(torch.float64)
+        r = (3 + steps) % 3
+        u64 = b[rots[r][0]][1:1 + sd, h64[1]:h64[1] + sd]
+        den = float(u64.abs().max())
+        err = max(err, float((u32 - u64).abs().max()) / den if den > 0 else 0.0)
+    g32 = cells * steps / (ms32 * 1e-3) / 1e9
+    g64v = cells * steps / (ms64 * 1e-3) / 1e9
+    out = {"instances": count, "cells": cells, "steps": steps,
+           "batched_fp32": {"ms_per_step": ms32 / steps, "gcell_updates_per_s": g32, "launches_per_step": 1},
+           "plain_fp64": {"ms_per_step": ms64 / steps, "gcell_updates_per_s": g64v, "launches_per_step": count},
+           "speedup": g32 / g64v, "rel_err_fp32_vs_fp64": err}
+    log(f"batched wave: fp32 batched {g32:.1f} vs fp64 plain {g64v:.1f} Gcell-updates/s "
+        f"= {g32 / g64v:.2f}x, rel err {err:.2e}")
+    return out
+
+
 def cpu_baseline(n=256, steps=8):
     """The NumPy oracle (one core: NumPy ufuncs are single-threaded) on a
     256^3 sample of the headline workload."""
@@ -217,8 +471,23 @@ def main():
                                        args.warmup, "f32", spec)
         extra["copy_ceiling"] = copy_ceiling(head["alg_bytes_per_launch"] // 2)
         log(f"copy ceiling: {extra['copy_ceiling']['gbs']:.0f} GB/s")
+        for key, fn in (("temporal_depths_c2", lambda: temporal_depths(interior, n)),):
+            try:
+                extra[key] = fn()
+            except Exception as e:  # an extra must not hide the headline line
+                extra[key] = {"error": f"{type(e).__name__}: {e}"}
+                log(f"{key} failed: {e}")
     del interior
     torch.cuda.empty_cache()
+    if not args.no_extra:
+        for key, fn in (("heat2d_c1", heat2d_c1), ("gray_scott_c3", gray_scott_c3),
+                        ("batched_wave_c4", batched_wave_c4)):
+            try:
+                extra[key] = fn()
+            except Exception as e:
+                extra[key] = {"error": f"{type(e).__name__}: {e}"}
+                log(f"{key} failed: {e}")
+            torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
