@@ -248,8 +248,8 @@ def gray_scott_c3(n=4096, steps=1000):
             f["c_u"], f["c_v"] = torch.zeros_like(f["u"]), torch.zeros_like(f["v"])
         targets = list(r.outputs) if upd else ["u", "v"]
         scratch = {t: torch.empty_like(f[t]) for t in targets}
-        stp = FusedStepper(r, spec, targets, fill=["u", "v"])
-        for t in ("u", "v"):
+        stp = FusedStepper(r, spec, targets)
+        for t in targets:
             from cutile.runtime import boundary
 
             boundary.fill(f[t], spec, r.halo_widths)

@@ -25,6 +25,7 @@ device result is bit-identical to NumPy on the same inputs.
 from __future__ import annotations
 
 import math
+import re
 import struct
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
@@ -353,8 +354,11 @@ def apply_update(meta: StencilMeta, mode: str, dt: float, targets: Sequence[str]
     ``c_<target>`` as an extra input and output,
     ``y = dt*R - c;  t = round_storage(u + y);  c' = (t - u) - y;  u' = t``,
     so the low-order part that the storage type drops is carried to the
-    next step (upstream's time steppers update in the array's precision,
-    ``rk_integrator.py:183-232``)."""
+    next step, and ``t - c'`` equals ``(u - c) + dt*R`` up to the rounding
+    of ``c'``.  The rates are evaluated on that compensated state ``u - c``
+    (every access of a target field reads both arrays), so the pair behaves
+    as one value with about twice the storage mantissa.  Upstream's time
+    steppers update in the array's precision (``rk_integrator.py:183-232``)."""
     if mode not in UPDATE_MODES:
         raise ValueError(f"update must be one of {UPDATE_MODES}, got {mode!r}")
     targets = list(targets)
@@ -380,6 +384,29 @@ def apply_update(meta: StencilMeta, mode: str, dt: float, targets: Sequence[str]
             meta.input_names.append(f"c_{t}")
             cf.append(len(meta.input_names) - 1)
             centre(cf[-1])
+        # the rates see the compensated state u - c at every access of a
+        # target field, so the stored pair (u, c) acts as one number with
+        # about twice the storage mantissa
+        comp_of = {f: c for f, c in zip(tf, cf)}
+        for a in list(meta.accesses):
+            if a.array_index in comp_of and (comp_of[a.array_index], tuple(a.offsets)) not in have:
+                c = comp_of[a.array_index]
+                meta.accesses.append(AccessInfo(c, tuple(a.offsets), _view_name(meta.input_names[c], a.offsets)))
+                have.add((c, tuple(a.offsets)))
+        pat = re.compile(r"a\.template g<(\d+), (-?\d+), (-?\d+), (-?\d+)>\(\)")
+
+        def comp_read(m):
+            f = int(m.group(1))
+            if f not in comp_of:
+                return m.group(0)
+            rest = f"{m.group(2)}, {m.group(3)}, {m.group(4)}"
+            return f"(a.template g<{f}, {rest}>() - a.template g<{comp_of[f]}, {rest}>())"
+
+        meta.expressions = [pat.sub(comp_read, e) for e in meta.expressions]
+        pyn = {meta.input_names[f]: meta.input_names[c] for f, c in comp_of.items()}
+        meta.py_expressions = [re.sub(r"\bt_(" + "|".join(pyn) + r")_([0-9mp_]+)",
+                                      lambda m: f"(t_{m.group(1)}_{m.group(2)} - t_{pyn[m.group(1)]}_{m.group(2)})", e)
+                               for e in meta.py_expressions]
     meta.update = {"mode": mode, "dt": float(dt), "tf": tf, "cf": cf, "targets": targets}
     meta.output_names = targets + ([f"c_{t}" for t in targets] if mode == "kahan" else [])
     return meta

@@ -41,10 +41,12 @@ def storage_compute(dtype: str, storage: Optional[str]) -> tuple:
     raise ValueError(f"unsupported storage {storage!r} for dtype {dtype!r}")
 
 
-def make_configs(ndim: int, tile: Sequence[int], storage: str, threads: Optional[int] = None) -> tuple:
+def make_configs(ndim: int, tile: Sequence[int], storage: str, threads: Optional[int] = None,
+                 vector: Optional[int] = None) -> tuple:
     """(vector config, scalar config, chunk) for a ``tile_sizes`` tuple;
     ``threads`` overrides the workgroup size (a multiple of the threads one
-    tile row needs)."""
+    tile row needs); ``vector`` the cells per thread along the contiguous
+    axis (default: one 16-byte vector; fewer cells hold fewer registers)."""
     tile = tuple(int(t) for t in tile)
     if ndim == 3:
         xch, ty, tz = tile
@@ -53,7 +55,7 @@ def make_configs(ndim: int, tile: Sequence[int], storage: str, threads: Optional
     else:
         (tz,), ty, xch = tile, 1, 1
     sb = ELEM_BYTES[storage]
-    vz = max(1, min(16 // sb, tz))
+    vz = max(1, min(int(vector) if vector else 16 // sb, 16 // sb, tz))
 
     def cfg(tz_, vz_):
         ntz = max(1, tz_ // vz_)
@@ -86,7 +88,7 @@ def fit_lds(meta: StencilMeta, tile: Sequence[int], hints: Optional[dict], stora
     hints = dict(hints or {})
     nd = len(tile)
     while True:
-        vec, _, _ = make_configs(nd, tile, storage_t, threads=hints.get("threads"))
+        vec, _, _ = make_configs(nd, tile, storage_t, threads=hints.get("threads"), vector=hints.get("vector"))
         if lds_bytes(meta, vec, compute) <= budget:
             return tile, hints
         if nd == 3 and tile[1] > 1:
@@ -111,7 +113,8 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
                 temporal_steps: int = 1, boundary_spec=None, kernel_hints: Optional[dict] = None,
                 storage: Optional[str] = None, fused: bool = False, temporal_mode: str = "zero") -> str:
     storage_t, compute_t = storage_compute(meta.dtype, storage)
-    vec, scal, xch = make_configs(meta.ndim, tile_sizes, storage_t, (kernel_hints or {}).get("threads"))
+    vec, scal, xch = make_configs(meta.ndim, tile_sizes, storage_t, (kernel_hints or {}).get("threads"),
+                                  (kernel_hints or {}).get("vector"))
     temporal = None
     if temporal_steps > 1:
         if meta.num_inputs != 1 or meta.num_outputs != 1:

@@ -109,11 +109,19 @@ class TilingPass:
         base = {"threads": nt} if nt != self.threads_per_block else {}
         if tier == "narrow":
             return [(first, dict(base))]
-        smaller = first if ndim < 3 else (first[0], max(1, first[1] // 2), first[2])
-        ladder = [(smaller, dict(base, occupancy=occ))
+        small = dict(base)
+        if ndim == 3:
+            smaller = (first[0], max(1, first[1] // 2), first[2])
+        elif self.vector > 1:
+            # 1-D / 2-D rows: half the cells per thread (half-width vectors)
+            smaller = first[:-1] + (first[-1] // 2,)
+            small["vector"] = self.vector // 2
+        else:
+            smaller = first
+        ladder = [(smaller, dict(small, occupancy=occ))
                   for occ in range(self.very_wide_occupancy, self.min_occupancy - 1, -1)]
-        ordered = [(first, dict(base)), (smaller, dict(base))] + ladder if tier == "wide" else \
-            [(first, dict(base))] + ladder + [(smaller, dict(base))]
+        ordered = [(first, dict(base)), (smaller, small)] + ladder if tier == "wide" else \
+            [(first, dict(base))] + ladder + [(smaller, small)]
         seen: list = []
         for c in ordered:
             if c not in seen:

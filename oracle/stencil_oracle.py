@@ -287,13 +287,18 @@ def time_update(fields: dict, rate_fns: Sequence[Callable], targets: Sequence[st
     ``R_k`` are the rate functions evaluated on the old fields (the compute
     type is the arrays' dtype; bfloat16 storage is held as float32 arrays of
     bf16 values).  ``"euler"`` stores ``round(u + dt*R)``; ``"kahan"`` keeps a
-    compensation per field: ``y = dt*R - c; t = round(u + y);
-    c = round((t - u) - y); u = t``."""
+    compensation per field, evaluates the rates on the compensated fields
+    ``u - c`` (whole arrays, halos included), and updates
+    ``y = dt*R - c; t = round(u + y); c = round((t - u) - y); u = t``."""
     shape = next(iter(fields.values())).shape
     dtype = next(iter(fields.values())).dtype
     I = interior(shape, halo)
     dtc = dtype.type(dt)
-    rates = [eval_multi(fields, fn, ndim, halo) for fn in rate_fns]
+    seen = dict(fields)
+    if mode == "kahan":
+        for t in targets:
+            seen[t] = fields[t] - comps[t]
+    rates = [eval_multi(seen, fn, ndim, halo) for fn in rate_fns]
     for t, R in zip(targets, rates):
         base = fields[t][I].copy()
         if mode == "euler":

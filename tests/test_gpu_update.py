@@ -18,7 +18,7 @@ def _run(r, fields, steps, spec, storage):
     dt = getattr(torch, storage)
     f = {n: to_dev(a.astype(np.float32) if storage != "float64" else a, dt) for n, a in fields.items()}
     scratch = {n: torch.zeros_like(f[n]) for n in r.outputs}
-    st = FusedStepper(r, spec, list(r.outputs), fill=["u", "v"])
+    st = FusedStepper(r, spec, list(r.outputs))
     f, _ = st.enqueue(f, scratch, steps)
     torch.cuda.synchronize()
     return {n: to_host(f[n]).astype(np.float64 if storage == "float64" else np.float32) for n in f}
@@ -33,6 +33,8 @@ def _oracle(oracle, fields, rate_fns, steps, spec, halo, mode, storage, dtype):
         oracle.time_update(f, rate_fns, ["u", "v"], 2, halo, 1.0, mode, comps, storage)
         for n in ("u", "v"):
             oracle.apply_boundary(f[n], spec, halo)
+            if mode == "kahan":
+                oracle.apply_boundary(comps[n], spec, halo)
     return f, comps
 
 

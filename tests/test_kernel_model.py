@@ -103,7 +103,8 @@ def test_random_stencil_launches_in_bounds(seed):
     meta = extract_meta(find_func(fn._ir))
     plan = plan_accesses(meta)
     acc = access_table(meta)
-    vec, scal, xch = make_configs(nd, r.tile_sizes, meta.dtype, threads=r.kernel_hints.get("threads"))
+    vec, scal, xch = make_configs(nd, r.tile_sizes, meta.dtype, threads=r.kernel_hints.get("threads"),
+                                  vector=r.kernel_hints.get("vector"))
     base = {1: (300,), 2: (37, 70), 3: (11, 13, 40)}[nd]
     shape = tuple(b + 2 * h for b, h in zip(base, r.halo_widths))
     if nd == 1:
