@@ -77,6 +77,9 @@ struct SweepArgs {
   int nxcd;         // XCDs for the workgroup swizzle (1 = identity)
   int tmode;        // temporal blocking: 0 = zero halo between levels, 1 = recompute ghosts
   int yslab;        // tile order: 0 = XCD gets an x-slab (z, y, x-chunk), 1 = a y-slab (z, x-chunk, y)
+  int wrap;         // bit d set: axis d is periodic and the sweep also writes the output's halo
+                    // along it (cells within h_d of a face are copied to the opposite halo;
+                    // requires n_d >= 2 h_d)
 };
 
 // Bijective remap of dispatch order -> logical tile so that XCD x (which
@@ -293,6 +296,34 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   }
   load_bufs(xb);
 
+  // periodic halo written by the sweep: a cell within h_d of a face of a
+  // wrapped axis is also stored at its image across the domain (every
+  // combination of wrapped axes, so edges and corners get the doubly and
+  // triply wrapped values a per-axis fill would give).  Images are halo cells
+  // of the output, written by no other cell, so no two threads collide.
+  auto store_wrapped = [&](int x, int y, const C (&res)[NOUT][VZ]) {
+    const bool wx = (a.wrap & 1) && (x < a.h0 || x >= a.n0 - a.h0);
+    const bool wy = (a.wrap & 2) && (y < a.h1 || y >= a.n1 - a.h1);
+    const int xm = x < a.h0 ? x + a.n0 : x - a.n0;
+    const int ym = y < a.h1 ? y + a.n1 : y - a.n1;
+#pragma unroll
+    for (int v = 0; v < VZ; ++v) {
+      const int z = zc + v;
+      if (z >= a.n2) continue;
+      const bool wz = (a.wrap & 4) && (z < a.h2 || z >= a.n2 - a.h2);
+      if (!(wx || wy || wz)) continue;
+      const int zm = z < a.h2 ? z + a.n2 : z - a.n2;
+#pragma unroll
+      for (int c = 1; c < 8; ++c) {
+        if (((c & 1) && !wx) || ((c & 2) && !wy) || ((c & 4) && !wz)) continue;
+        const i64 X = (c & 1) ? xm : x, Y = (c & 2) ? ym : y, Z = (c & 4) ? zm : z;
+        const i64 o = ((a.h0 + X) * s1 + (a.h1 + Y)) * s2 + a.h2 + Z;
+#pragma unroll
+        for (int m = 0; m < NOUT; ++m) reinterpret_cast<S*>(a.out[m])[o] = Cvt<S, C>::st(res[m][v]);
+      }
+    }
+  };
+
   // one plane: rotation R = (x - xb) % QN names the queue slots, so the
   // queue never moves between registers
   auto step = [&](auto Rc, int x) {
@@ -384,6 +415,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
             if (zc + v < a.n2) dst[v] = Cvt<S, C>::st(res[m][v]);
         }
       }
+      if (a.wrap) store_wrapped(x, y, res);
     }
   };
 

@@ -53,7 +53,7 @@ class SweepArgs(ctypes.Structure):
         ("n0", ctypes.c_int), ("n1", ctypes.c_int), ("n2", ctypes.c_int),
         ("ntz", ctypes.c_int), ("nty", ctypes.c_int), ("nch", ctypes.c_int),
         ("xch", ctypes.c_int), ("nxcd", ctypes.c_int), ("tmode", ctypes.c_int),
-        ("yslab", ctypes.c_int),
+        ("yslab", ctypes.c_int), ("wrap", ctypes.c_int),
     ]
 
 
@@ -255,14 +255,38 @@ class StencilKernel:
             stream = default_stream()
         rt.launch(fn, (nblocks, 1, 1), (self.variants["temporal"][4], 1, 1), args, stream)
 
+    def wrap_mask(self, periodic_axes: Sequence[int], args) -> int:
+        """Bits of the 3-D map for the periodic array axes; every such axis
+        needs an interior at least twice its halo."""
+        if self.ndim == 1:
+            bits = {0: 4}
+        elif self.ndim == 2:
+            bits = {0: 1, 1: 4}
+        else:
+            bits = {0: 1, 1: 2, 2: 4}
+        mask = 0
+        for ax in periodic_axes:
+            b = bits[int(ax)]
+            n, h = {1: (args.n0, args.h0), 2: (args.n1, args.h1), 4: (args.n2, args.h2)}[b]
+            if n < 2 * h:
+                raise ValueError(f"periodic halo in the sweep needs interior >= 2*halo on axis {ax} "
+                                 f"(interior {n}, halo {h})")
+            mask |= b
+        return mask
+
     def launch(self, inputs: Sequence, outputs: Sequence, halo: Sequence[int], stream=None,
-               xch: Optional[int] = None, validate: bool = True) -> None:
+               xch: Optional[int] = None, validate: bool = True, periodic_axes: Sequence[int] = ()) -> None:
+        """One sweep.  ``periodic_axes``: array axes whose output halo the
+        sweep writes as well (periodic wrap), so no halo fill is needed
+        between steps for them."""
         if validate:
             self.check_arrays(inputs, outputs)
         key = self.pick(inputs, outputs, halo)
         args, nblocks = self.args(inputs, outputs, halo, key, xch)
         if args is None:
             return
+        if periodic_axes:
+            args.wrap = self.wrap_mask(periodic_axes, args)
         fn = self._function(key)
         if stream is None:
             stream = default_stream()

@@ -31,11 +31,36 @@ def _bc_kind(spec, ndim) -> str:
     for lo, hi in spec.values():
         for s in (lo, hi):
             kinds.add(s if isinstance(s, str) else (s[0] if s[1] == 0.0 else "dirichlet_value"))
-    if kinds == {"periodic"}:
+    if kinds == {"periodic"} and len(spec) == ndim:
         return "periodic"
-    if kinds == {"dirichlet"}:
+    if kinds == {"dirichlet"} and len(spec) == ndim:
         return "dirichlet0"
+    if kinds <= {"dirichlet", "dirichlet_value"} and len(spec) == ndim:
+        return "dirichlet"
     return "other"
+
+
+class HaloPolicy:
+    """How the halo is kept valid between steps:
+
+    * ``"wrap"``: all axes periodic -- the sweep writes the output's halo
+      itself (``periodic_axes``), no fill launches;
+    * ``"static"``: every side a Dirichlet value -- the sweep never writes
+      halo cells, so both buffers are filled once and stay valid;
+    * ``"fill"``: anything else -- a halo fill after every step."""
+
+    def __init__(self, spec, ndim: int, halo):
+        kind = _bc_kind(spec, ndim)
+        self.mode = {"periodic": "wrap", "dirichlet0": "static", "dirichlet": "static"}.get(kind, "fill")
+        self.ndim = ndim
+        self.halo = tuple(halo)
+
+    def axes_for(self, u) -> tuple:
+        """Periodic axes the sweep may wrap for array ``u`` (interior >= 2*halo)."""
+        if self.mode != "wrap":
+            return ()
+        ok = all(s - 2 * h >= 2 * h for s, h in zip(u.shape, self.halo))
+        return tuple(range(self.ndim)) if ok else ()
 
 
 class TimeStepper:
@@ -68,6 +93,7 @@ class TimeStepper:
         self._block = self.block.load_module()._KERNEL if self.block else None
         self.ndim = self.single.ndim
         self.storage = self.single.storage
+        self.policy = HaloPolicy(boundary, self.ndim, self.halo)
 
     # -- arrays ---------------------------------------------------------
     def empty_like_interior(self, interior_shape, device="cuda"):
@@ -80,9 +106,10 @@ class TimeStepper:
         _boundary.fill(u, self.spec, self.halo, stream)
 
     # -- stepping -------------------------------------------------------
-    def _one(self, a, b, stream):
-        self._single.launch([a], [b], self.halo, stream, validate=False)
-        self.fill(b, stream)
+    def _one(self, a, b, stream, axes=()):
+        self._single.launch([a], [b], self.halo, stream, validate=False, periodic_axes=axes)
+        if not axes and self.policy.mode != "static":
+            self.fill(b, stream)
         return b, a
 
     def _blk(self, a, b, stream):
@@ -99,12 +126,15 @@ class TimeStepper:
             from cutile.runtime.kernel import default_stream
 
             stream = default_stream()
+        if self.policy.mode == "static":
+            self.fill(b, stream)          # constant halo: valid in both buffers from here on
         if self._block is not None:
             for _ in range(steps // self.T):
                 a, b = self._blk(a, b, stream)
             steps %= self.T
+        axes = self.policy.axes_for(a)
         for _ in range(steps):
-            a, b = self._one(a, b, stream)
+            a, b = self._one(a, b, stream, axes)
         return a, b
 
     def march(self, u, steps: int, scratch=None, stream=None, graph: bool = False, graph_steps: int = 0):
@@ -173,18 +203,25 @@ class FusedStepper:
         self.fill_names = list(fill) if fill is not None else list(targets)
         self.halo = result.halo_widths
         self._k = result.load_module()._KERNEL
+        self.policy = HaloPolicy(boundary, result.ndim, self.halo)
 
     def enqueue(self, fields: dict, scratch: dict, steps: int, stream=None):
         from cutile.runtime.kernel import default_stream
 
         if stream is None:
             stream = default_stream()
+        if self.policy.mode == "static":
+            for t in self.fill_names:
+                _boundary.fill(scratch[t], self.spec, self.halo, stream)
+        axes = self.policy.axes_for(fields[self.targets[0]])
         for _ in range(steps):
             ins = [fields[n] for n in self.r.inputs]
             outs = [scratch[t] for t in self.targets]
-            self._k.launch(ins, outs, self.halo, stream, validate=False)
-            for t in self.fill_names:
-                _boundary.fill(scratch[t], self.spec, self.halo, stream)
+            # a wrapped sweep writes every output's periodic halo
+            self._k.launch(ins, outs, self.halo, stream, validate=False, periodic_axes=axes)
+            if not axes and self.policy.mode != "static":
+                for t in self.fill_names:
+                    _boundary.fill(scratch[t], self.spec, self.halo, stream)
             for t in self.targets:
                 fields[t], scratch[t] = scratch[t], fields[t]
         return fields, scratch

@@ -47,7 +47,7 @@ def queue_planes(xb, xe, qlo, qhi, pd):
     return planes
 
 
-def check_launch(cfg, plan, shape3, halo3, n3, xch, yslab, nxcd=8, accesses=None, tiles="edges", pd=1):
+def check_launch(cfg, plan, shape3, halo3, n3, xch, yslab, nxcd=8, accesses=None, tiles="edges", pd=1, wrap=0):
     """Return a list of out-of-bounds descriptions (empty when clean)."""
     s0, s1, s2 = shape3
     h0, h1, h2 = halo3
@@ -144,6 +144,36 @@ def check_launch(cfg, plan, shape3, halo3, n3, xch, yslab, nxcd=8, accesses=None
                 off = ((h0 + x) * s1 + (h1 + y)) * s2 + h2 + zc
                 gcheck(f"output store plane {x}", off, m)
                 gcheck(f"output store plane {x} (vector end)", off + VZ - 1, m & (zc + VZ <= n2))
+                if wrap:
+                    # images of cells near a face of a wrapped axis (st::sweep_tile store_wrapped)
+                    wx = bool(wrap & 1) and (x < h0 or x >= n0 - h0)
+                    xm = x + n0 if x < h0 else x - n0
+                    wy = (wrap & 2) > 0
+                    wy = wy & ((y < h1) | (y >= n1 - h1))
+                    ym = np.where(y < h1, y + n1, y - n1)
+                    for v in range(VZ):
+                        z = zc + v
+                        mz = m & (z < n2)
+                        wz = bool(wrap & 4) & ((z < h2) | (z >= n2 - h2))
+                        zm = np.where(z < h2, z + n2, z - n2)
+                        for c in range(1, 8):
+                            sel = mz.copy()
+                            if c & 1:
+                                if not wx:
+                                    continue
+                            if c & 2:
+                                sel = sel & wy
+                            if c & 4:
+                                sel = sel & wz
+                            X = xm if c & 1 else x
+                            Y = ym if c & 2 else y
+                            Z = zm if c & 4 else z
+                            o = ((h0 + X) * s1 + (h1 + Y)) * s2 + h2 + Z
+                            gcheck(f"wrapped store c={c} plane {x}", o, sel)
+                            # an image must land in the halo, never in the interior
+                            inside = ((X >= 0) & (X < n0)) & ((Y >= 0) & (Y < n1)) & ((Z >= 0) & (Z < n2))
+                            if np.any(np.broadcast_to(inside, sel.shape) & sel):
+                                errs.append(f"wrapped store c={c} plane {x} hits the interior")
         if len(errs) > 20:
             return errs
     return errs

@@ -119,3 +119,24 @@ def test_random_stencil_launches_in_bounds(seed):
         for x in (xch, 4, 1):
             errs = km.check_launch(cfg, plan, s3, h3, n3, x, 0, accesses=acc)
             assert not errs, errs[:5]
+
+
+@pytest.mark.parametrize("n", [(512, 512, 512), (8, 8, 40), (37, 45, 70), (9, 10, 33)])
+def test_wrapped_periodic_stores_in_bounds(n):
+    """Images written by a periodic (wrapped) sweep stay inside the array and
+    inside the halo, for interiors down to twice the halo."""
+    for fn in (w.diffusion7(), w.star25()):
+        meta = extract_meta(find_func(fn._ir))
+        plan = plan_accesses(meta)
+        h = max(abs(o) for a in meta.accesses for o in a.offsets)
+        t, hints = TilingPass(dtype_bytes=8).candidates(3, len(access_table(meta)))[0]
+        vec, scal, xch = make_configs(3, t, meta.dtype, threads=hints.get("threads"))
+        halo3 = (h, h, 16)
+        if min(nn - 2 * hh for nn, hh in zip(n, halo3)) < 0:
+            continue
+        shape3 = tuple(k + 2 * hh for k, hh in zip(n, halo3))
+        for cfg in (vec, scal):
+            if shape3[2] % cfg.vz:
+                continue
+            errs = km.check_launch(cfg, plan, shape3, halo3, n, xch, 0, accesses=access_table(meta), wrap=7)
+            assert not errs, errs[:5]
