@@ -296,34 +296,6 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   }
   load_bufs(xb);
 
-  // periodic halo written by the sweep: a cell within h_d of a face of a
-  // wrapped axis is also stored at its image across the domain (every
-  // combination of wrapped axes, so edges and corners get the doubly and
-  // triply wrapped values a per-axis fill would give).  Images are halo cells
-  // of the output, written by no other cell, so no two threads collide.
-  auto store_wrapped = [&](int x, int y, const C (&res)[NOUT][VZ]) {
-    const bool wx = (a.wrap & 1) && (x < a.h0 || x >= a.n0 - a.h0);
-    const bool wy = (a.wrap & 2) && (y < a.h1 || y >= a.n1 - a.h1);
-    const int xm = x < a.h0 ? x + a.n0 : x - a.n0;
-    const int ym = y < a.h1 ? y + a.n1 : y - a.n1;
-#pragma unroll
-    for (int v = 0; v < VZ; ++v) {
-      const int z = zc + v;
-      if (z >= a.n2) continue;
-      const bool wz = (a.wrap & 4) && (z < a.h2 || z >= a.n2 - a.h2);
-      if (!(wx || wy || wz)) continue;
-      const int zm = z < a.h2 ? z + a.n2 : z - a.n2;
-#pragma unroll
-      for (int c = 1; c < 8; ++c) {
-        if (((c & 1) && !wx) || ((c & 2) && !wy) || ((c & 4) && !wz)) continue;
-        const i64 X = (c & 1) ? xm : x, Y = (c & 2) ? ym : y, Z = (c & 4) ? zm : z;
-        const i64 o = ((a.h0 + X) * s1 + (a.h1 + Y)) * s2 + a.h2 + Z;
-#pragma unroll
-        for (int m = 0; m < NOUT; ++m) reinterpret_cast<S*>(a.out[m])[o] = Cvt<S, C>::st(res[m][v]);
-      }
-    }
-  };
-
   // one plane: rotation R = (x - xb) % QN names the queue slots, so the
   // queue never moves between registers
   auto step = [&](auto Rc, int x) {
@@ -393,29 +365,57 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
         }
       }
       const i64 off = ((i64)(a.h0 + x) * s1 + (a.h1 + y)) * s2 + a.h2 + zc;
+      using W = typename NtVec<S, VZ>::type;
+      if (zc + VZ <= a.n2) {
+        W w[NOUT];
 #pragma unroll
-      for (int m = 0; m < NOUT; ++m) {
-        S* dst = reinterpret_cast<S*>(a.out[m]) + off;
-        if (zc + VZ <= a.n2) {
-          typename NtVec<S, VZ>::type w;
+        for (int m = 0; m < NOUT; ++m) {
           if constexpr (VZ == 1) {
-            w = Cvt<S, C>::st(res[m][0]);
+            w[m] = Cvt<S, C>::st(res[m][0]);
           } else {
 #pragma unroll
-            for (int v = 0; v < VZ; ++v) w[v] = Cvt<S, C>::st(res[m][v]);
+            for (int v = 0; v < VZ; ++v) w[m][v] = Cvt<S, C>::st(res[m][v]);
           }
+          W* dst = reinterpret_cast<W*>(reinterpret_cast<S*>(a.out[m]) + off);
           if constexpr (P::NTS) {
-            __builtin_nontemporal_store(w, reinterpret_cast<typename NtVec<S, VZ>::type*>(dst));
+            __builtin_nontemporal_store(w[m], dst);
           } else {
-            *reinterpret_cast<typename NtVec<S, VZ>::type*>(dst) = w;
+            *dst = w[m];
           }
-        } else {
+        }
+        // periodic halo written by the sweep: a vector within h_d of a face
+        // of a wrapped axis is also stored at its images across the domain
+        // (every combination of wrapped axes, so edges and corners get the
+        // values a per-axis fill would give).  A vector variant runs only
+        // when n2 and h2 are multiples of VZ, so z-images are whole aligned
+        // vectors; images are halo cells written by no other thread.
+        if (a.wrap) {
+          const bool wx = (a.wrap & 1) && (x < a.h0 || x >= a.n0 - a.h0);
+          const bool wy = (a.wrap & 2) && (y < a.h1 || y >= a.n1 - a.h1);
+          const bool wz = (a.wrap & 4) && (zc < a.h2 || zc >= a.n2 - a.h2);
+          if (wx || wy || wz) {
+            const i64 xm = x < a.h0 ? x + a.n0 : x - a.n0;
+            const i64 ym = y < a.h1 ? y + a.n1 : y - a.n1;
+            const i64 zm = zc < a.h2 ? zc + a.n2 : zc - a.n2;
+#pragma unroll 1
+            for (int c = 1; c < 8; ++c) {
+              if (((c & 1) && !wx) || ((c & 2) && !wy) || ((c & 4) && !wz)) continue;
+              const i64 X = (c & 1) ? xm : x, Y = (c & 2) ? ym : y, Z = (c & 4) ? zm : zc;
+              const i64 o = ((a.h0 + X) * s1 + (a.h1 + Y)) * s2 + a.h2 + Z;
+#pragma unroll
+              for (int m = 0; m < NOUT; ++m) *reinterpret_cast<W*>(reinterpret_cast<S*>(a.out[m]) + o) = w[m];
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < NOUT; ++m) {
+          S* dst = reinterpret_cast<S*>(a.out[m]) + off;
 #pragma unroll
           for (int v = 0; v < VZ; ++v)
             if (zc + v < a.n2) dst[v] = Cvt<S, C>::st(res[m][v]);
         }
       }
-      if (a.wrap) store_wrapped(x, y, res);
     }
   };
 

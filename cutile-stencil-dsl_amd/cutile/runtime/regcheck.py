@@ -62,7 +62,9 @@ def parse_notes(text: str) -> dict:
             return int(mm.group(1)) if mm else 0
 
         vgpr = num("vgpr_count") + num("agpr_count")
-        spill = num("vgpr_spill_count") + num("sgpr_spill_count")
+        # VGPR spills go to scratch memory; SGPR spills go to VGPR lanes
+        # (v_writelane/v_readlane) and use no memory, so they do not count
+        spill = num("vgpr_spill_count")
         scratch = num("private_segment_fixed_size")
         out[m.group(1)] = Resources(vgpr, max(scratch, 4 * spill), None, num("group_segment_fixed_size"))
     return out
