@@ -77,9 +77,8 @@ struct SweepArgs {
   int nxcd;         // XCDs for the workgroup swizzle (1 = identity)
   int tmode;        // temporal blocking: 0 = zero halo between levels, 1 = recompute ghosts
   int yslab;        // tile order: 0 = XCD gets an x-slab (z, y, x-chunk), 1 = a y-slab (z, x-chunk, y)
-  int wrap;         // bit d set: axis d is periodic and the sweep also writes the output's halo
-                    // along it (cells within h_d of a face are copied to the opposite halo;
-                    // requires n_d >= 2 h_d)
+  int wrap;         // bit d set: axis d is periodic; reads past the interior wrap around
+                    // (the halo along it is neither read nor written)
 };
 
 // Bijective remap of dispatch order -> logical tile so that XCD x (which
@@ -169,12 +168,68 @@ template <class P, int R> struct Cell {
   }
 };
 
-template <class S, int V> struct NtVec { typedef S type __attribute__((ext_vector_type(V))); };
-template <class S> struct NtVec<S, 1> { typedef S type; };
+// ---------------------------------------------------------------------------
+// buffer memory access
+// ---------------------------------------------------------------------------
+//
+// Every global load and store of the sweep goes through a buffer resource
+// covering one plane (base = the plane, range = its bytes).  A lane whose
+// access must not happen gets the offset BUF_OOB instead of a branch: the
+// hardware drops such stores and returns 0 for such loads.  With no
+// branches around memory instructions their count per step is fixed, so the
+// compiler waits only for the loads it needs (vmcnt(N)) and the stores of
+// the previous plane stay in flight.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int BUF_OOB = (int)0x80000000;          // beyond any plane's range (planes < 2 GiB)
+constexpr int BUF_CONFIG = 0x00020000;            // raw buffer, dword data format (gfx9 family)
+constexpr int BUF_NT = 2;                         // non-temporal store
+
+ST_HD Rsrc make_rsrc(const void* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, BUF_CONFIG);
+}
+
+template <int B> struct BufIO;
+template <> struct BufIO<16> {
+  typedef unsigned T __attribute__((ext_vector_type(4)));
+  static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0); }
+  template <int AUX> static ST_HD void st(T v, Rsrc r, int off) { __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, AUX); }
+};
+template <> struct BufIO<8> {
+  typedef unsigned T __attribute__((ext_vector_type(2)));
+  static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0); }
+  template <int AUX> static ST_HD void st(T v, Rsrc r, int off) { __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, AUX); }
+};
+template <> struct BufIO<4> {
+  typedef unsigned T;
+  static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0); }
+  template <int AUX> static ST_HD void st(T v, Rsrc r, int off) { __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, AUX); }
+};
+template <> struct BufIO<2> {
+  typedef unsigned short T;
+  static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0); }
+  template <int AUX> static ST_HD void st(T v, Rsrc r, int off) { __builtin_amdgcn_raw_buffer_store_b16(v, r, off, 0, AUX); }
+};
+
+template <class S, int V> ST_HD Vec<S, V> bload(Rsrc r, int off) {
+  using IO = BufIO<(int)sizeof(S) * V>;
+  return __builtin_bit_cast(Vec<S, V>, IO::ld(r, off));
+}
+template <class S, int V, int AUX> ST_HD void bstore(const Vec<S, V>& x, Rsrc r, int off) {
+  using IO = BufIO<(int)sizeof(S) * V>;
+  IO::template st<AUX>(__builtin_bit_cast(typename IO::T, x), r, off);
+}
+
+// index along a periodic axis of interior extent n (|i| < n beyond either side)
+ST_HD int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // ---------------------------------------------------------------------------
 // the plane-streaming sweep
 // ---------------------------------------------------------------------------
+//
+// a.wrap bit d: axis d of the 3-D map is periodic.  Reads past the interior
+// along it wrap around instead of reading the halo, so no halo fill is
+// needed between steps (the output's halo is not written).  The host allows
+// it when n_d >= h_d and, for the vector variant, n2 % VZ == 0.
 template <class P>
 ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   using S = typename P::S;
@@ -183,6 +238,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   constexpr int NT = P::NT, TY = P::TY, TZ = P::TZ, VZ = P::VZ, NF = P::NF, NO = P::NO, NB = P::NB;
   constexpr int NOUT = P::NOUT;  // arrays written: NO results (+ NO compensations)
   constexpr int NTZ = T::NTZ, NTY = T::NTY, KY = T::KY, QN = T::QN, NVB = T::max_nvb();
+  constexpr int ES = (int)sizeof(S);
   static_assert(P::NT % NTZ == 0, "NT must be a multiple of TZ/VZ");
 
   __shared__ C lds[T::lds_elems()];
@@ -192,60 +248,62 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   const int z0 = tile_z * TZ, y0 = tile_y * TY;
   const int xb = chunk * a.xch;
   const int xe = (xb + a.xch < a.n0) ? xb + a.xch : a.n0;
-  const i64 s1 = a.s1, s2 = a.s2;
+  const int s1 = (int)a.s1, s2 = (int)a.s2;
+  const i64 plane = a.s1 * a.s2;
+  const int pbytes = (int)(plane * ES);
+  const bool w0 = a.wrap & 1, w1 = a.wrap & 2, w2 = a.wrap & 4;
   const int zc = z0 + tz * VZ;                  // first interior z of my vector
   const bool zin = zc < a.n2;                   // my cells are interior (stored)
-  const bool zload = a.h2 + zc + VZ <= s2;      // my vector lies inside the array
+  const int zq = w2 ? wrapi(zc, a.n2) : zc;     // where my vector is read from
+  const bool zload = a.h2 + zq + VZ <= s2;      // ... and it lies inside the array
 
   constexpr int PD = P::PD;   // planes prefetched ahead into registers
   C q[NF][QN][KY][VZ];
   C pq[PD][NF][KY][VZ];
   C pb[NB > 0 ? NB : 1][NVB][VZ];
 
-  auto plane_ptr = [&](int f, int x) -> const S* {
-    return reinterpret_cast<const S*>(a.in[f]) + (i64)(a.h0 + x) * s1 * s2;
+  auto in_rsrc = [&](int f, int x) -> Rsrc {
+    const int px = w0 ? wrapi(x, a.n0) : x;
+    return make_rsrc(reinterpret_cast<const S*>(a.in[f]) + (i64)(a.h0 + px) * plane, pbytes);
   };
 
   // my cells of plane x of field f (cells outside the interior but inside
   // the array are loaded too: they are the halo other cells read)
-  auto load_cells = [&](int f, int x, C (&dst)[KY][VZ]) {
-    const S* p = plane_ptr(f, x);
+  // (live = false issues the same loads out of range: zeros, no traffic)
+  auto load_cells = [&](int f, int x, C (&dst)[KY][VZ], bool live) {
+    const Rsrc r = in_rsrc(f, x);
 #pragma unroll
     for (int k = 0; k < KY; ++k) {
       const int y = y0 + ty + k * NTY;
-      if (zload && a.h1 + y < s1 && ty + k * NTY < TY) {
-        Vec<S, VZ> w = vload<S, VZ>(p + (i64)(a.h1 + y) * s2 + a.h2 + zc);
+      const int yq = w1 ? wrapi(y, a.n1) : y;
+      const bool ok = live && zload && a.h1 + yq < s1 && ty + k * NTY < TY;
+      const Vec<S, VZ> w = bload<S, VZ>(r, ok ? ((a.h1 + yq) * s2 + a.h2 + zq) * ES : BUF_OOB);
 #pragma unroll
-        for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
-      } else {
-#pragma unroll
-        for (int v = 0; v < VZ; ++v) dst[k][v] = C(0);
-      }
+      for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
     }
   };
 
   // prefetch the memory part (halo ring, or whole tile) of every LDS buffer
-  auto load_bufs = [&](int x) {
+  auto load_bufs = [&](int x, bool live) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const S* p = plane_ptr(P::BF[b], x + P::BDX[b]);
+      const Rsrc r = in_rsrc(P::BF[b], x + P::BDX[b]);
 #pragma unroll
       for (int j = 0; j < NVB; ++j) {
         const int i = tid + j * NT;
-        bool ok = (j < T::nvb(b)) && (i < T::gvecs(b));
-        int r = 0, c = 0;
-        T::gvec_rc(b, ok ? i : 0, r, c);
-        const i64 gy = (i64)a.h1 + y0 - P::BRY[b] + r;
-        const i64 gz = (i64)a.h2 + z0 - T::rzv(b) + c * VZ;
+        bool ok = live && (j < T::nvb(b)) && (i < T::gvecs(b));
+        int rr = 0, c = 0;
+        T::gvec_rc(b, ok ? i : 0, rr, c);
+        int gy = y0 - P::BRY[b] + rr;           // interior coordinates
+        int gz = z0 - T::rzv(b) + c * VZ;
+        if (w1) gy = wrapi(gy, a.n1);
+        if (w2) gz = wrapi(gz, a.n2);
+        gy += a.h1;
+        gz += a.h2;
         ok = ok && gy >= 0 && gy < s1 && gz >= 0 && gz + VZ <= s2;
-        if (ok) {
-          Vec<S, VZ> w = vload<S, VZ>(p + gy * s2 + gz);
+        const Vec<S, VZ> w = bload<S, VZ>(r, ok ? (gy * s2 + gz) * ES : BUF_OOB);
 #pragma unroll
-          for (int v = 0; v < VZ; ++v) pb[b][j][v] = Cvt<S, C>::ld(w.v[v]);
-        } else {
-#pragma unroll
-          for (int v = 0; v < VZ; ++v) pb[b][j][v] = C(0);
-        }
+        for (int v = 0; v < VZ; ++v) pb[b][j][v] = Cvt<S, C>::ld(w.v[v]);
       }
     }
   };
@@ -288,13 +346,12 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     for (int f = 0; f < NF; ++f) {
       if (!P::QF[f]) continue;
 #pragma unroll
-      for (int i = 0; i + 1 < QN; ++i) load_cells(f, xb + P::QLO + i, q[f][i]);
+      for (int i = 0; i + 1 < QN; ++i) load_cells(f, xb + P::QLO + i, q[f][i], true);
 #pragma unroll
-      for (int k = 0; k < PD; ++k)
-        if (xb + k < xe) load_cells(f, xb + P::QHI + k, pq[k][f]);
+      for (int k = 0; k < PD; ++k) load_cells(f, xb + P::QHI + k, pq[k][f], xb + k < xe);
     }
   }
-  load_bufs(xb);
+  load_bufs(xb, true);
 
   // one plane: rotation R = (x - xb) % QN names the queue slots, so the
   // queue never moves between registers
@@ -323,21 +380,25 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       __syncthreads();
     }
     // prefetch plane x+PD of the queue and the LDS ring of plane x+1 while
-    // plane x is computed
+    // plane x is computed (a plane past the chunk is loaded as out of range)
     if constexpr (T::HASQ) {
-      if (x + PD < xe) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f)
-          if (P::QF[f]) load_cells(f, x + PD + P::QHI, pq[PD - 1][f]);
-      }
+      for (int f = 0; f < NF; ++f)
+        if (P::QF[f]) load_cells(f, x + PD + P::QHI, pq[PD - 1][f], x + PD < xe);
     }
-    if (x + 1 < xe) load_bufs(x + 1);
+    load_bufs(x + 1, x + 1 < xe);
 
+    Rsrc ro[NOUT];
+#pragma unroll
+    for (int m = 0; m < NOUT; ++m)
+      ro[m] = make_rsrc(reinterpret_cast<S*>(a.out[m]) + (i64)(a.h0 + x) * plane, pbytes);
 #pragma unroll
     for (int k = 0; k < KY; ++k) {
-      const int lyk = ty + k * NTY;
-      const int y = y0 + lyk;
-      if (lyk >= TY || y >= a.n1 || !zin) continue;
+      // rows past the tile (when NTY does not divide TY) compute on the
+      // last row's LDS data and store nothing
+      const int lyk = ty + k * NTY < TY ? ty + k * NTY : TY - 1;
+      const int y = y0 + ty + k * NTY;
+      const bool valid = ty + k * NTY < TY && y < a.n1 && zin;
       C res[NOUT][VZ];
 #pragma unroll
       for (int v = 0; v < VZ; ++v) {
@@ -364,57 +425,27 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
           }, typename mkseq<NO>::type{});
         }
       }
-      const i64 off = ((i64)(a.h0 + x) * s1 + (a.h1 + y)) * s2 + a.h2 + zc;
-      using W = typename NtVec<S, VZ>::type;
-      if (zc + VZ <= a.n2) {
-        W w[NOUT];
+      const int off = ((a.h1 + y) * s2 + a.h2 + zc) * ES;
+      if (!P::PARTIAL || zc + VZ <= a.n2 || !zin) {
+        // whole vectors: one unconditional store per output
 #pragma unroll
         for (int m = 0; m < NOUT; ++m) {
-          if constexpr (VZ == 1) {
-            w[m] = Cvt<S, C>::st(res[m][0]);
-          } else {
+          Vec<S, VZ> w;
 #pragma unroll
-            for (int v = 0; v < VZ; ++v) w[m][v] = Cvt<S, C>::st(res[m][v]);
-          }
-          W* dst = reinterpret_cast<W*>(reinterpret_cast<S*>(a.out[m]) + off);
-          if constexpr (P::NTS) {
-            __builtin_nontemporal_store(w[m], dst);
-          } else {
-            *dst = w[m];
-          }
-        }
-        // periodic halo written by the sweep: a vector within h_d of a face
-        // of a wrapped axis is also stored at its images across the domain
-        // (every combination of wrapped axes, so edges and corners get the
-        // values a per-axis fill would give).  A vector variant runs only
-        // when n2 and h2 are multiples of VZ, so z-images are whole aligned
-        // vectors; images are halo cells written by no other thread.
-        if (a.wrap) {
-          const bool wx = (a.wrap & 1) && (x < a.h0 || x >= a.n0 - a.h0);
-          const bool wy = (a.wrap & 2) && (y < a.h1 || y >= a.n1 - a.h1);
-          const bool wz = (a.wrap & 4) && (zc < a.h2 || zc >= a.n2 - a.h2);
-          if (wx || wy || wz) {
-            const i64 xm = x < a.h0 ? x + a.n0 : x - a.n0;
-            const i64 ym = y < a.h1 ? y + a.n1 : y - a.n1;
-            const i64 zm = zc < a.h2 ? zc + a.n2 : zc - a.n2;
-#pragma unroll 1
-            for (int c = 1; c < 8; ++c) {
-              if (((c & 1) && !wx) || ((c & 2) && !wy) || ((c & 4) && !wz)) continue;
-              const i64 X = (c & 1) ? xm : x, Y = (c & 2) ? ym : y, Z = (c & 4) ? zm : zc;
-              const i64 o = ((a.h0 + X) * s1 + (a.h1 + Y)) * s2 + a.h2 + Z;
-#pragma unroll
-              for (int m = 0; m < NOUT; ++m) *reinterpret_cast<W*>(reinterpret_cast<S*>(a.out[m]) + o) = w[m];
-            }
-          }
+          for (int v = 0; v < VZ; ++v) w.v[v] = Cvt<S, C>::st(res[m][v]);
+          bstore<S, VZ, P::NTS ? BUF_NT : 0>(w, ro[m], valid ? off : BUF_OOB);
         }
       } else {
+        // the ragged end of a row whose interior is not a multiple of VZ
+        // (batched instances with padded rows)
 #pragma unroll
-        for (int m = 0; m < NOUT; ++m) {
-          S* dst = reinterpret_cast<S*>(a.out[m]) + off;
+        for (int m = 0; m < NOUT; ++m)
 #pragma unroll
-          for (int v = 0; v < VZ; ++v)
-            if (zc + v < a.n2) dst[v] = Cvt<S, C>::st(res[m][v]);
-        }
+          for (int v = 0; v < VZ; ++v) {
+            Vec<S, 1> e;
+            e.v[0] = Cvt<S, C>::st(res[m][v]);
+            bstore<S, 1, 0>(e, ro[m], valid && zc + v < a.n2 ? off + v * ES : BUF_OOB);
+          }
       }
     }
   };

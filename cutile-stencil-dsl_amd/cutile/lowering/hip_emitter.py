@@ -100,7 +100,10 @@ def temporal_tile(ndim: int, T: int, r3: tuple, compute: str, budget: int = 160 
 
 
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,
-                nontemporal: bool = True, prefetch: int = 1) -> str:
+                nontemporal: bool = True, prefetch: int = 1, partial: bool = False) -> str:
+    """Traits struct of one kernel variant.  ``partial``: rows may end in a
+    partial vector (batched instances whose interior is not a multiple of
+    the vector width)."""
     plan = plan_accesses(meta)
     bufs = plan["bufs"]
     r3 = radius3(meta)
@@ -118,7 +121,8 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int BRY[{max(1, len(bufs))}] = {_arr([b[2] for b in bufs])};",
         f"  static constexpr int BRZ[{max(1, len(bufs))}] = {_arr([b[3] for b in bufs])};",
         f"  static constexpr int RX = {r3[0]}, RY = {r3[1]}, RZ = {r3[2]};",
-        f"  static constexpr int NTS = {1 if nontemporal else 0}, PD = {int(prefetch)};",
+        f"  static constexpr int NTS = {1 if nontemporal else 0}, PD = {int(prefetch)}, "
+        f"PARTIAL = {1 if partial else 0};",
     ]
     upd = meta.update
     ne = meta.num_exprs

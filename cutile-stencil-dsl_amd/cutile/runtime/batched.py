@@ -106,7 +106,7 @@ class BatchedStencil:
                 raise ValueError(f"bucket {b}: {ntz} threads per row must divide 256")
             cfg = KernelConfig(ty=b.ty, tz=b.tz, vz=self.vz, nt=256)
             self.cfgs.append(cfg)
-            parts.append(emit_traits(meta, cfg, self.storage, self.compute, f"b{k}"))
+            parts.append(emit_traits(meta, cfg, self.storage, self.compute, f"b{k}", partial=True))
             cases.append(f"      case {k}: st::sweep_tile<P_b{k}>(a, tz, ty, ch); break;")
         self.symbol = f"{self.name}_batched"
         parts.append("struct BatchItem { int inst, tz, ty, cb; };")

@@ -223,6 +223,8 @@ class StencilKernel:
             a.inp[k] = t.data_ptr()
         for k, t in enumerate(outputs):
             a.out[k] = t.data_ptr()
+        if s[1] * s[2] * inputs[0].element_size() >= 1 << 31:
+            raise ValueError(f"a plane of {s[1]}x{s[2]} cells exceeds the 2 GiB range of one buffer resource")
         a.s0, a.s1, a.s2 = s
         a.h0, a.h1, a.h2 = h
         a.n0, a.n1, a.n2 = n
@@ -256,8 +258,9 @@ class StencilKernel:
         rt.launch(fn, (nblocks, 1, 1), (self.variants["temporal"][4], 1, 1), args, stream)
 
     def wrap_mask(self, periodic_axes: Sequence[int], args) -> int:
-        """Bits of the 3-D map for the periodic array axes; every such axis
-        needs an interior at least twice its halo."""
+        """Bits of the 3-D map for the periodic array axes (the sweep wraps
+        reads past the interior instead of reading the halo); every such
+        axis needs an interior at least as wide as its halo."""
         if self.ndim == 1:
             bits = {0: 4}
         elif self.ndim == 2:
@@ -268,17 +271,18 @@ class StencilKernel:
         for ax in periodic_axes:
             b = bits[int(ax)]
             n, h = {1: (args.n0, args.h0), 2: (args.n1, args.h1), 4: (args.n2, args.h2)}[b]
-            if n < 2 * h:
-                raise ValueError(f"periodic halo in the sweep needs interior >= 2*halo on axis {ax} "
+            if n < h:
+                raise ValueError(f"periodic reads in the sweep need interior >= halo on axis {ax} "
                                  f"(interior {n}, halo {h})")
             mask |= b
         return mask
 
     def launch(self, inputs: Sequence, outputs: Sequence, halo: Sequence[int], stream=None,
                xch: Optional[int] = None, validate: bool = True, periodic_axes: Sequence[int] = ()) -> None:
-        """One sweep.  ``periodic_axes``: array axes whose output halo the
-        sweep writes as well (periodic wrap), so no halo fill is needed
-        between steps for them."""
+        """One sweep.  ``periodic_axes``: array axes along which the sweep
+        reads periodically (past the interior it wraps around instead of
+        reading the halo), so no halo fill is needed between steps; the
+        output's halo is not written."""
         if validate:
             self.check_arrays(inputs, outputs)
         key = self.pick(inputs, outputs, halo)

@@ -43,8 +43,9 @@ def _bc_kind(spec, ndim) -> str:
 class HaloPolicy:
     """How the halo is kept valid between steps:
 
-    * ``"wrap"``: all axes periodic -- the sweep writes the output's halo
-      itself (``periodic_axes``), no fill launches;
+    * ``"wrap"``: all axes periodic -- the sweep reads periodically past
+      the interior (``periodic_axes``), so no fill between steps; one fill
+      at the end leaves the result's halo as a per-step fill would;
     * ``"static"``: every side a Dirichlet value -- the sweep never writes
       halo cells, so both buffers are filled once and stay valid;
     * ``"fill"``: anything else -- a halo fill after every step."""
@@ -56,10 +57,10 @@ class HaloPolicy:
         self.halo = tuple(halo)
 
     def axes_for(self, u) -> tuple:
-        """Periodic axes the sweep may wrap for array ``u`` (interior >= 2*halo)."""
+        """Periodic axes the sweep may wrap for array ``u`` (interior >= halo)."""
         if self.mode != "wrap":
             return ()
-        ok = all(s - 2 * h >= 2 * h for s, h in zip(u.shape, self.halo))
+        ok = all(s - 2 * h >= h for s, h in zip(u.shape, self.halo))
         return tuple(range(self.ndim)) if ok else ()
 
 
@@ -135,6 +136,8 @@ class TimeStepper:
         axes = self.policy.axes_for(a)
         for _ in range(steps):
             a, b = self._one(a, b, stream, axes)
+        if axes and steps:
+            self.fill(a, stream)          # the wrapped sweeps left the halo stale
         return a, b
 
     def march(self, u, steps: int, scratch=None, stream=None, graph: bool = False, graph_steps: int = 0):
@@ -217,11 +220,13 @@ class FusedStepper:
         for _ in range(steps):
             ins = [fields[n] for n in self.r.inputs]
             outs = [scratch[t] for t in self.targets]
-            # a wrapped sweep writes every output's periodic halo
             self._k.launch(ins, outs, self.halo, stream, validate=False, periodic_axes=axes)
             if not axes and self.policy.mode != "static":
                 for t in self.fill_names:
                     _boundary.fill(scratch[t], self.spec, self.halo, stream)
             for t in self.targets:
                 fields[t], scratch[t] = scratch[t], fields[t]
+        if axes and steps:
+            for t in self.fill_names:     # the wrapped sweeps left the halos stale
+                _boundary.fill(fields[t], self.spec, self.halo, stream)
         return fields, scratch

@@ -3,8 +3,13 @@
 Mirrors, index for index, ``st::sweep_tile`` in ``csrc/stencil_core.h``
 (register-queue loads, LDS halo-ring fetches, LDS stores and reads, output
 stores) for a given variant, array shape and launch, and reports every
-access that would leave its buffer.  Used by the CPU tests to prove a
-launch configuration in bounds before it runs on a device."""
+access that would leave its buffer.  Global accesses go through one buffer
+resource per plane: the plane itself must lie in the array and a live
+access must stay inside the plane (accesses the kernel marks out of range
+are dropped by the hardware and are not checked).  ``wrap`` bit d maps reads
+past the interior along axis d around it, as the kernel does.  Used by the
+CPU tests to prove a launch configuration in bounds before it runs on a
+device."""
 
 from __future__ import annotations
 
@@ -68,6 +73,18 @@ def check_launch(cfg, plan, shape3, halo3, n3, xch, yslab, nxcd=8, accesses=None
         if idx.size and (idx.min() < 0 or idx.max() >= numel):
             errs.append(f"{what}: index range [{idx.min()}, {idx.max()}] outside [0, {numel})")
 
+    def wrapi(i, n, on):
+        i = np.asarray(i)
+        return np.where(i < 0, i + n, np.where(i >= n, i - n, i)) if on else i
+
+    def pcheck(what, x, rowcol, mask):
+        """plane x (interior coords) inside the array, offsets inside the plane"""
+        idx = np.asarray(rowcol)[np.broadcast_to(mask, np.shape(rowcol))]
+        if idx.size and (idx.min() < 0 or idx.max() >= s1 * s2):
+            errs.append(f"{what}: in-plane offset range [{idx.min()}, {idx.max()}] outside [0, {s1 * s2})")
+        if np.any(mask) and not (0 <= h0 + x < s0):
+            errs.append(f"{what}: plane {x} outside the array")
+
     def lcheck(what, idx, mask):
         idx = np.asarray(idx)[mask]
         if idx.size and (idx.min() < 0 or idx.max() >= t["lds"]):
@@ -106,74 +123,43 @@ def check_launch(cfg, plan, shape3, halo3, n3, xch, yslab, nxcd=8, accesses=None
         zc = z0 + tz * VZ
         zin = zc < n2
         zload = h2 + zc + VZ <= s2
-        # the extreme planes the queue touches (prologue and prefetches)
-        if hasq:
-            ql = queue_planes(xb, xe, qlo, qhi, pd)
-            qplanes = (min(ql), max(ql))
-        else:
-            qplanes = None
+        # every plane the queue loads (prologue and prefetches)
+        qpl = queue_planes(xb, xe, qlo, qhi, pd) if hasq else []
+        zq = wrapi(zc, n2, wrap & 4)
+        zload_q = h2 + zq + VZ <= s2
         for k in range(KY):
             y = y0 + ty + k * NTY
-            m = zload & (h1 + y < s1) & (ty + k * NTY < TY)
-            if qplanes:
-                for x in qplanes:
-                    base = (h0 + x) * s1 * s2 + (h1 + y) * s2 + h2 + zc
-                    gcheck(f"queue load plane {x}", base, m)
-                    gcheck(f"queue load plane {x} (vector end)", base + VZ - 1, m)
+            yq = wrapi(y, n1, wrap & 2)
+            m = zload_q & (h1 + yq < s1) & (ty + k * NTY < TY)
+            for x in (min(qpl), max(qpl)) if qpl else ():
+                px = int(wrapi(x, n0, wrap & 1))
+                pcheck(f"queue load plane {x}", px, (h1 + yq) * s2 + h2 + zq, m)
+                pcheck(f"queue load plane {x} (vector end)", px, (h1 + yq) * s2 + h2 + zq + VZ - 1, m)
         for bi, b in enumerate(t["bufs"]):
             r_, c_, m_ = rc[bi]
-            gy = h1 + y0 - b["ry"] + r_
-            gz = h2 + z0 - b["rzv"] + c_ * VZ
+            gy = h1 + wrapi(y0 - b["ry"] + r_, n1, wrap & 2)
+            gz = h2 + wrapi(z0 - b["rzv"] + c_ * VZ, n2, wrap & 4)
             ok = m_ & (gy >= 0) & (gy < s1) & (gz >= 0) & (gz + VZ <= s2)
             for x in (xb + b["dx"], xe - 1 + b["dx"]):
-                p = (h0 + x) * s1 * s2
-                gcheck(f"halo fetch plane {x}", p + gy * s2 + gz, ok)
-                gcheck(f"halo fetch plane {x} (vector end)", p + gy * s2 + gz + VZ - 1, ok)
+                px = int(wrapi(x, n0, wrap & 1))
+                pcheck(f"halo fetch plane {x}", px, gy * s2 + gz, ok)
+                pcheck(f"halo fetch plane {x} (vector end)", px, gy * s2 + gz + VZ - 1, ok)
         for k in range(KY):
             lyk = ty + k * NTY
             y = y0 + lyk
             m = (lyk < TY) & (y < n1) & zin
+            lyc = np.minimum(lyk, TY - 1)       # rows past the tile compute on the last row
             for (f, dx, dy, dz) in (accesses or []):
                 if dy == 0 and dz == 0 and hasq and qlo <= dx <= qhi:
                     continue
                 b = next(bb for bb in t["bufs"] if bb["f"] == f and bb["dx"] == dx)
                 for v in range(VZ):
-                    idx = b["off"] + (lyk + b["ry"] + dy) * b["cols"] + (tz * VZ + v + b["rzv"] + dz)
-                    lcheck("LDS read", idx, m)
+                    idx = b["off"] + (lyc + b["ry"] + dy) * b["cols"] + (tz * VZ + v + b["rzv"] + dz)
+                    lcheck("LDS read", idx, np.ones_like(m))
             for x in (xb, xe - 1):
-                off = ((h0 + x) * s1 + (h1 + y)) * s2 + h2 + zc
-                gcheck(f"output store plane {x}", off, m)
-                gcheck(f"output store plane {x} (vector end)", off + VZ - 1, m & (zc + VZ <= n2))
-                if wrap:
-                    # images of cells near a face of a wrapped axis (st::sweep_tile store_wrapped)
-                    wx = bool(wrap & 1) and (x < h0 or x >= n0 - h0)
-                    xm = x + n0 if x < h0 else x - n0
-                    wy = (wrap & 2) > 0
-                    wy = wy & ((y < h1) | (y >= n1 - h1))
-                    ym = np.where(y < h1, y + n1, y - n1)
-                    for v in range(VZ):
-                        z = zc + v
-                        mz = m & (z < n2)
-                        wz = bool(wrap & 4) & ((z < h2) | (z >= n2 - h2))
-                        zm = np.where(z < h2, z + n2, z - n2)
-                        for c in range(1, 8):
-                            sel = mz.copy()
-                            if c & 1:
-                                if not wx:
-                                    continue
-                            if c & 2:
-                                sel = sel & wy
-                            if c & 4:
-                                sel = sel & wz
-                            X = xm if c & 1 else x
-                            Y = ym if c & 2 else y
-                            Z = zm if c & 4 else z
-                            o = ((h0 + X) * s1 + (h1 + Y)) * s2 + h2 + Z
-                            gcheck(f"wrapped store c={c} plane {x}", o, sel)
-                            # an image must land in the halo, never in the interior
-                            inside = ((X >= 0) & (X < n0)) & ((Y >= 0) & (Y < n1)) & ((Z >= 0) & (Z < n2))
-                            if np.any(np.broadcast_to(inside, sel.shape) & sel):
-                                errs.append(f"wrapped store c={c} plane {x} hits the interior")
+                pcheck(f"output store plane {x}", x, (h1 + y) * s2 + h2 + zc, m)
+                pcheck(f"output store plane {x} (vector end)", x, (h1 + y) * s2 + h2 + zc + VZ - 1,
+                       m & (zc + VZ <= n2))
         if len(errs) > 20:
             return errs
     return errs

@@ -1,6 +1,7 @@
 """Halo policies of the steppers against the oracle, whole arrays (halo
-included), bit for bit: the periodic halo written by the sweep itself
-("wrap"), Dirichlet halos filled once ("static"), and the per-step fill."""
+included), bit for bit: periodic reads wrapped inside the sweep with one
+fill at the end ("wrap"), Dirichlet halos filled once ("static"), and the
+per-step fill."""
 
 import numpy as np
 import pytest
@@ -52,9 +53,9 @@ def test_wrap_star25_fp32(oracle, shape):
 def test_narrow_periodic_falls_back_to_fills(oracle):
     from cutile import workloads as w
 
-    print("periodic star25 fp64 interior < 2*halo on axis 0 (fill path)", flush=True)
-    st = _stepper_vs_oracle(oracle, w.star25(), (6, 12, 40), w.periodic_spec(3), 3)
-    assert st.policy.axes_for(st.empty_like_interior((6, 12, 40))) == ()
+    print("periodic star25 fp64 interior < halo on axis 0 (fill path)", flush=True)
+    st = _stepper_vs_oracle(oracle, w.star25(), (3, 12, 40), w.periodic_spec(3), 3)
+    assert st.policy.axes_for(st.empty_like_interior((3, 12, 40))) == ()
 
 
 def test_wrap_heat1d_and_2d(oracle):
