@@ -50,12 +50,21 @@ def test_wrap_star25_fp32(oracle, shape):
     _stepper_vs_oracle(oracle, w.star25("float32"), shape, w.periodic_spec(3), 4, dtype="float32", expect="wrap")
 
 
-def test_narrow_periodic_falls_back_to_fills(oracle):
+def test_periodic_interior_as_narrow_as_the_halo(oracle):
     from cutile import workloads as w
 
-    print("periodic star25 fp64 interior < halo on axis 0 (fill path)", flush=True)
-    st = _stepper_vs_oracle(oracle, w.star25(), (3, 12, 40), w.periodic_spec(3), 3)
-    assert st.policy.axes_for(st.empty_like_interior((3, 12, 40))) == ()
+    print("periodic star25 fp64 interior == halo on axis 0 (wrap path)", flush=True)
+    st = _stepper_vs_oracle(oracle, w.star25(), (4, 12, 40), w.periodic_spec(3), 3, expect="wrap")
+    assert st.policy.axes_for(st.empty_like_interior((4, 12, 40))) == (0, 1, 2)
+
+
+def test_neumann_uses_a_fill_per_step(oracle):
+    from cutile import workloads as w
+
+    print("neumann star25 fp64 (fill path)", flush=True)
+    spec = {d: ("neumann", "neumann") for d in range(3)}
+    st = _stepper_vs_oracle(oracle, w.star25(), (9, 12, 40), spec, 3, expect="fill")
+    assert st.policy.axes_for(st.empty_like_interior((9, 12, 40))) == ()
 
 
 def test_wrap_heat1d_and_2d(oracle):
