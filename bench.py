@@ -56,9 +56,11 @@ def traffic_from_profiles(kernel: str):
     return None
 
 
-def run_steps(launch, fill, a, b, steps, stream, rt, record_kernel=True):
-    """Run ``steps`` steps (sweep a->b, halo fill b, swap); returns
-    (total_ms, mean sweep ms, a, b)."""
+def run_steps(launch, a, b, steps, stream, rt, record_kernel=True):
+    """Run ``steps`` steps (sweep a->b, swap); returns (total_ms, mean sweep
+    ms, a, b).  Each step is one sweep kernel: the periodic boundary is
+    handled inside it (reads wrap around the interior), so no halo fill runs
+    between steps."""
     starts = [rt.Event() for _ in range(steps)] if record_kernel else []
     ends = [rt.Event() for _ in range(steps)] if record_kernel else []
     e0, e1 = rt.Event(), rt.Event()
@@ -69,7 +71,6 @@ def run_steps(launch, fill, a, b, steps, stream, rt, record_kernel=True):
         launch(a, b)
         if record_kernel:
             ends[k].record(stream)
-        fill(b)
         a, b = b, a
     e1.record(stream)
     total = e0.elapsed_ms(e1)
@@ -91,18 +92,20 @@ def measure(name, stencil_fn, interior_dev, n, steps, warmup, dtype_name, spec, 
     a = torch.zeros(shape, dtype=tdt, device="cuda")
     a[tuple(slice(h, h + n) for h in halo)] = interior_dev.to(tdt)
     b = torch.zeros_like(a)
-    launch = r.launcher
-    fill = (lambda t: boundary.fill(t, spec, halo))
-    fill(a)
+    boundary.fill(a, spec, halo)
+    K = r.load_module()._KERNEL
     s = default_stream()
+
+    def launch(x, y):
+        K.launch([x], [y], halo, s, validate=False, periodic_axes=(0, 1, 2))
+
     log(f"{name}: compile + warmup ({warmup} steps), halo {halo}, tile {r.tile_sizes}")
-    _, _, a, b = run_steps(launch, fill, a, b, max(1, warmup), s, rt, record_kernel=False)
+    _, _, a, b = run_steps(launch, a, b, max(1, warmup), s, rt, record_kernel=False)
     torch.cuda.synchronize()
-    total_ms, kern_ms, a, b = run_steps(launch, fill, a, b, steps, s, rt)
+    total_ms, kern_ms, a, b = run_steps(launch, a, b, steps, s, rt)
     torch.cuda.synchronize()
     cells = n ** 3
     esz = a.element_size()
-    K = r.load_module()._KERNEL
     key = K.pick([a], [b], halo)
     res = K.resources(key)
     out = {
@@ -221,11 +224,12 @@ def heat2d_c1(steps=100, n=1024):
 def gray_scott_c3(n=4096, steps=1000):
     """Fused Gray-Scott 4096^2, periodic, 1000 steps: float64, float32,
     bfloat16 storage with a Kahan-compensated update (float32 arithmetic),
-    float32 with a Kahan update.  Errors after all steps, against float64
-    runs: from the exact (float64) initial state, and from the same initial
-    state rounded to the variant's storage -- the pattern-forming dynamics
-    amplify the rounding of the 1 % noise itself, which the second figure
-    separates from the arithmetic and storage error."""
+    float32 with a Kahan update.  Errors per field after all steps, against
+    float64 runs from the exact (float64) initial state and from the same
+    initial state rounded to the variant's storage.  With these parameters
+    the 64 seeded squares die out: v decays to ~1e-41 (subnormal) by step
+    1000, so v's relative error measures subnormal resolution; its absolute
+    error and u's relative error are the meaningful figures."""
     import torch
 
     from cutile import compile_fused
@@ -282,16 +286,22 @@ def gray_scott_c3(n=4096, steps=1000):
         del f, scratch, g, stp
         torch.cuda.empty_cache()
 
-    def err(a, b):
-        return max(float((a[t] - b[t]).abs().max() / b[t].abs().max()) for t in ("u", "v"))
+    def err(a, b, t):
+        return float((a[t] - b[t]).abs().max() / b[t].abs().max())
 
+    ref = finals["float64"]
+    out["float64"]["max_abs_u"] = float(ref["u"].abs().max())
+    out["float64"]["max_abs_v"] = float(ref["v"].abs().max())
     for name, same in (("float32", "float64_from_float32_init"), ("float32_kahan", "float64_from_float32_init"),
                        ("bfloat16_kahan", "float64_from_bfloat16_init")):
-        out[name]["rel_err_vs_float64"] = err(finals[name], finals["float64"])
-        out[name]["rel_err_vs_float64_same_init"] = err(finals[name], finals[same])
-        log(f"gray-scott {name}: rel max err after {steps} steps vs float64 = "
-            f"{out[name]['rel_err_vs_float64']:.3e}, vs float64 from the same rounded init = "
-            f"{out[name]['rel_err_vs_float64_same_init']:.3e}")
+        for t in ("u", "v"):
+            out[name][f"rel_err_{t}_vs_float64"] = err(finals[name], ref, t)
+            out[name][f"rel_err_{t}_vs_float64_same_init"] = err(finals[name], finals[same], t)
+            out[name][f"abs_err_{t}_vs_float64"] = float((finals[name][t] - ref[t]).abs().max())
+        log(f"gray-scott {name}: after {steps} steps rel max err u {out[name]['rel_err_u_vs_float64']:.3e} "
+            f"(same init {out[name]['rel_err_u_vs_float64_same_init']:.3e}), v "
+            f"{out[name]['rel_err_v_vs_float64']:.3e} (abs {out[name]['abs_err_v_vs_float64']:.2e}; "
+            f"max|v| of the float64 run {out['float64']['max_abs_v']:.2e})")
     return out
 
 
