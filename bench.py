@@ -417,6 +417,28 @@ def batched_wave_c4(count=512, steps=200, seed=0):
     return out
 
 
+def serving_c5(seconds=10.0, rate=200.0, seed=0):
+    """The serving stream (Poisson 200 jobs/s, mixed heat / 3-D diffusion /
+    Gray-Scott, 80 % float32) for ``seconds`` of arrivals: p50/p99 latency
+    and aggregate Gcell-updates/s; every class's kernels are built before
+    the clock starts."""
+    from cutile.serving import StencilServer, generate_jobs
+
+    srv = StencilServer()
+    warm = generate_jobs(50.0, 0.2, seed + 1)
+    for j in warm:
+        j.steps = 2
+    srv.run(warm, realtime=False)
+    jobs = generate_jobs(rate, seconds, seed)
+    stats = srv.run(jobs, realtime=True, log=log)
+    stats.update(rate=rate, seconds=seconds, offered_jobs=len(jobs),
+                 offered_gcell_updates_per_s=sum(j.cells * j.steps for j in jobs) / seconds / 1e9)
+    log(f"serving {seconds:.0f}s @ {rate:.0f} jobs/s: {stats['jobs']} jobs, p50 {stats['p50_ms']:.1f} ms, "
+        f"p99 {stats['p99_ms']:.1f} ms, {stats['gcell_updates_per_s']:.1f} Gcell-updates/s "
+        f"(offered {stats['offered_gcell_updates_per_s']:.1f})")
+    return stats
+
+
 def cpu_baseline(n=256, steps=8):
     """The NumPy oracle (one core: NumPy ufuncs are single-threaded) on a
     256^3 sample of the headline workload."""
@@ -499,7 +521,7 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_extra:
         for key, fn in (("heat2d_c1", heat2d_c1), ("gray_scott_c3", gray_scott_c3),
-                        ("batched_wave_c4", batched_wave_c4)):
+                        ("batched_wave_c4", batched_wave_c4), ("serving_c5", serving_c5)):
             try:
                 extra[key] = fn()
             except Exception as e:

@@ -1,0 +1,429 @@
+"""Continuous-batching stencil server on one MI355X.
+
+There is no upstream counterpart (upstream launches one kernel per array per
+step); this is the request loop of the serving configuration:
+
+* a job is a time-stepping request -- a stencil kind (2-D heat, 3-D 7-point
+  diffusion ``u += 0.1*L(u)``, fused Gray-Scott), a storage type, a side
+  length and a step count -- arriving at some time;
+* **admission by free HBM**: a job enters when its resident fields (two
+  buffers per field, halos included) fit in what the budget has left;
+  arrivals wait in FIFO order until completions free memory;
+* **resident fields**: an admitted job's buffers are allocated once and
+  initialised on the device from ``seed = job id``; the job steps in place
+  (ping-pong buffers) until done, then its buffers are released;
+* **packing**: every tick advances every active job by one step -- all jobs
+  of one (stencil, storage) class go through ONE launch of the batched
+  persistent kernel (``cutile.runtime.batched``) over a device work-list of
+  their tiles, with per-size-class tile buckets; periodic halos of a class
+  are refilled by one batched fill launch per axis; Dirichlet halos are
+  zero once and stay so (the sweep writes interiors only);
+* **continuous batching**: jobs join at the next tick after they are
+  admitted and leave after their last step, so work of jobs that arrived at
+  different times overlaps on the device; the work-lists are rebuilt only
+  when the active set changes (two plans per set: buffer parities flip
+  every tick);
+* latency = completion (the tick of a job's last step has finished on the
+  device) minus arrival; the loop reports p50/p99 latency and aggregate
+  Gcell-updates/s.
+
+Each job's result depends only on its own fields: the batched kernel
+evaluates every instance with the same arithmetic as the single-instance
+kernel, so a job gives the same bits alone and in any mix.
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+KINDS = ("heat2d", "diffusion3d", "grayscott")
+
+
+@dataclass
+class Job:
+    id: int
+    kind: str
+    dtype: str
+    side: int
+    steps: int
+    arrival: float = 0.0          # seconds after the start of the run
+    # runtime state
+    bufs: dict = field(default_factory=dict, repr=False)
+    done: int = 0
+    admitted: Optional[float] = None
+    finished: Optional[float] = None
+    nbytes: int = 0
+
+    @property
+    def ndim(self) -> int:
+        return 3 if self.kind == "diffusion3d" else 2
+
+    @property
+    def cells(self) -> int:
+        return self.side ** self.ndim
+
+    @property
+    def fields(self) -> tuple:
+        return ("u", "v") if self.kind == "grayscott" else ("u",)
+
+
+def generate_jobs(rate: float = 200.0, seconds: float = 60.0, seed: int = 0, fp32_share: float = 0.8,
+                  sides_2d=(128, 4096), sides_3d=(32, 384), steps=(10, 1000)) -> List[Job]:
+    """Poisson arrivals at ``rate`` jobs/s for ``seconds`` (seeded): kind
+    uniform over ``KINDS``, 2-D sides log-uniform in ``sides_2d``, 3-D sides
+    log-uniform in ``sides_3d``, steps uniform in ``steps``, float32 with
+    probability ``fp32_share`` else float64."""
+    rng = np.random.default_rng(seed)
+    t, jobs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        kind = KINDS[int(rng.integers(0, len(KINDS)))]
+        lo, hi = sides_3d if kind == "diffusion3d" else sides_2d
+        side = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        nsteps = int(rng.integers(steps[0], steps[1] + 1))
+        dtype = "float32" if rng.random() < fp32_share else "float64"
+        jobs.append(Job(len(jobs), kind, dtype, side, nsteps, t))
+    return jobs
+
+
+# ---------------------------------------------------------------------------
+# one (stencil, storage) class
+# ---------------------------------------------------------------------------
+
+class _Class:
+    def __init__(self, kind: str, dtype: str):
+        from cutile import workloads as w
+        from cutile.runtime.batched import BatchedStencil, Bucket
+
+        self.kind, self.dtype = kind, dtype
+        if kind == "heat2d":
+            fn = w.heat2d(dtype)
+        elif kind == "diffusion3d":
+            fn = w.diffusion7(dtype, coef=0.1)
+        else:
+            fn = list(w.gray_scott(dtype))
+        probe = BatchedStencil(fn)
+        vz = probe.vz
+        if kind == "diffusion3d":
+            buckets = [Bucket("rows", 4, 32 * vz, 16, 96), Bucket("rows", 8, 64 * vz, 32, None)]
+        else:
+            buckets = None                       # the 2-D plane/rows buckets by row length
+        self.bs = BatchedStencil(fn, buckets=buckets)
+        self.vz = vz
+        self.ndim = 3 if kind == "diffusion3d" else 2
+        self.halo = (1,) * (self.ndim - 1) + (vz,)   # contiguous-axis halo one vector wide
+        self.periodic = kind == "grayscott"
+        self.fields = ("u", "v") if kind == "grayscott" else ("u",)
+        self.esize = 4 if dtype == "float32" else 8
+        self.jobs: List[Job] = []
+        self.plans = None
+        self.fills = None
+        self.tick = 0
+
+    def shape(self, side: int) -> tuple:
+        cols = -(-side // self.vz) * self.vz             # rows padded to whole vectors
+        inner = (side,) * (self.ndim - 1) + (cols,)
+        return tuple(n + 2 * h for n, h in zip(inner, self.halo))
+
+    def job_bytes(self, side: int) -> int:
+        return 2 * len(self.fields) * int(np.prod(self.shape(side))) * self.esize
+
+    # -- buffers ----------------------------------------------------------------
+    def allocate(self, job: Job) -> None:
+        import torch
+
+        tdt = getattr(torch, self.dtype)
+        shp = self.shape(job.side)
+        job.bufs = {f: [torch.zeros(shp, dtype=tdt, device="cuda") for _ in range(2)] for f in self.fields}
+        job.nbytes = self.job_bytes(job.side)
+        init_fields(job, self, job.bufs)
+
+    # -- plans -------------------------------------------------------------------
+    def rebuild(self) -> None:
+        """Work-lists (and periodic fill tables) for the active set at both
+        buffer parities."""
+        self.plans, self.fills = [], []
+        for flip in (0, 1):
+            inst, interiors = [], []
+            for j in self.jobs:
+                p = (j.done + flip) % 2
+                ins = [j.bufs[f][p] for f in self.fields]
+                outs = [j.bufs[f][1 - p] for f in self.fields]
+                inst.append((ins, outs))
+                interiors.append((j.side,) * self.ndim)
+            self.plans.append(self.bs.prepare(inst, self.halo, interiors) if inst else None)
+            self.fills.append(self._fill_table(flip) if (inst and self.periodic) else None)
+
+    def _fill_table(self, flip: int):
+        import torch
+
+        from cutile.runtime import rt
+        from cutile.runtime.kernel import dtype_code
+
+        descs = (rt.FillDesc * (len(self.jobs) * len(self.fields)))()
+        k, max_slab = 0, 1
+        for j in self.jobs:
+            p = (j.done + flip) % 2
+            for f in self.fields:
+                t = j.bufs[f][1 - p]
+                d = descs[k]
+                d.ptr = t.data_ptr()
+                logical = tuple(j.side + 2 * h for h in self.halo)
+                for a in range(self.ndim):
+                    d.shape[a] = logical[a]
+                    d.halo[a] = self.halo[a]
+                    d.kinds[2 * a] = d.kinds[2 * a + 1] = rt.BC_CODES["periodic"]
+                d.ndim, d.dtype, d.pitch = self.ndim, dtype_code(t), t.shape[-1]
+                for a in range(self.ndim):
+                    slab = 2 * self.halo[a] * int(np.prod([logical[b] for b in range(self.ndim) if b != a]))
+                    max_slab = max(max_slab, slab)
+                k += 1
+        blob = torch.frombuffer(bytearray(bytes(descs)), dtype=torch.uint8).to("cuda")
+        return blob, k, max_slab
+
+    def step(self, stream) -> None:
+        """Advance every active job of the class by one step."""
+        from cutile.runtime import rt
+
+        flip = self.tick % 2
+        plan = self.plans[flip]
+        if plan is None:
+            return
+        self.bs.launch(plan, stream)
+        if self.fills[flip] is not None:
+            blob, count, max_slab = self.fills[flip]
+            rt.fill_halo_batched(blob.data_ptr(), count, self.ndim, max_slab, stream)
+        self.tick += 1
+        for j in self.jobs:
+            j.done += 1
+
+
+def init_fields(job: Job, cls: _Class, bufs: dict) -> None:
+    """Initial fields on the device from ``seed = job.id`` (the generators of
+    the benchmark configurations): heat -- uniform[0,1); 3-D diffusion --
+    three seeded sinusoids + 1e-3 uniform noise; Gray-Scott -- u = 1, v = 0,
+    64 seeded 10x10 squares (u = 0.5, v = 0.25) + 1 % uniform noise.  The
+    periodic Gray-Scott halo is filled once here."""
+    import torch
+
+    from cutile.runtime import boundary
+
+    g = torch.Generator(device="cuda").manual_seed(int(job.id))
+    rng = np.random.default_rng(int(job.id))
+    s, h = job.side, cls.halo
+    sl = tuple(slice(hh, hh + s) for hh in h)
+    if job.kind == "heat2d":
+        bufs["u"][0][sl] = torch.rand((s, s), generator=g, device="cuda", dtype=torch.float64).to(bufs["u"][0].dtype)
+    elif job.kind == "diffusion3d":
+        x = torch.arange(s, device="cuda", dtype=torch.float64) * (2.0 * math.pi / s)
+        acc = torch.zeros((s, s, s), device="cuda", dtype=torch.float64)
+        for _ in range(3):
+            k = rng.integers(-8, 9, size=3)
+            ph = float(rng.uniform(0, 2 * math.pi))
+            acc += torch.sin(k[0] * x[:, None, None] + k[1] * x[None, :, None] + k[2] * x[None, None, :] + ph)
+        acc += 1e-3 * torch.rand((s, s, s), generator=g, device="cuda", dtype=torch.float64)
+        bufs["u"][0][sl] = acc.to(bufs["u"][0].dtype)
+    else:
+        u = torch.ones((s, s), device="cuda", dtype=torch.float64)
+        v = torch.zeros((s, s), device="cuda", dtype=torch.float64)
+        side = min(10, s)
+        for _ in range(64):
+            cx, cy = (int(c) for c in rng.integers(0, s - side + 1, size=2))
+            u[cx:cx + side, cy:cy + side] = 0.5
+            v[cx:cx + side, cy:cy + side] = 0.25
+        u += 0.01 * torch.rand((s, s), generator=g, device="cuda", dtype=torch.float64)
+        v += 0.01 * torch.rand((s, s), generator=g, device="cuda", dtype=torch.float64)
+        bufs["u"][0][sl] = u.to(bufs["u"][0].dtype)
+        bufs["v"][0][sl] = v.to(bufs["v"][0].dtype)
+        for f in ("u", "v"):
+            _fill_padded(bufs[f][0], s, h)
+
+
+def _fill_padded(t, side: int, halo) -> None:
+    """Periodic fill of an array whose rows are padded past the logical
+    width (one batched-fill descriptor with a row pitch)."""
+    import torch
+
+    from cutile.runtime import rt
+    from cutile.runtime.kernel import default_stream, dtype_code
+
+    nd = t.dim()
+    d = rt.FillDesc()
+    d.ptr = t.data_ptr()
+    logical = tuple(side + 2 * h for h in halo)
+    max_slab = 1
+    for a in range(nd):
+        d.shape[a], d.halo[a] = logical[a], halo[a]
+        d.kinds[2 * a] = d.kinds[2 * a + 1] = rt.BC_CODES["periodic"]
+        max_slab = max(max_slab, 2 * halo[a] * int(np.prod([logical[b] for b in range(nd) if b != a])))
+    d.ndim, d.dtype, d.pitch = nd, dtype_code(t), t.shape[-1]
+    blob = torch.frombuffer(bytearray(bytes(d)), dtype=torch.uint8).to("cuda")
+    rt.fill_halo_batched(blob.data_ptr(), 1, nd, max_slab, default_stream())
+    torch.cuda.current_stream().synchronize()    # the descriptor blob is freed on return
+
+
+# ---------------------------------------------------------------------------
+# the server
+# ---------------------------------------------------------------------------
+
+class StencilServer:
+    """Admit, pack and step time-stepping jobs until all are done.
+
+    ``hbm_budget``: bytes of resident fields allowed at once (default: 90 %
+    of the device's free memory at start).  ``keep_results``: keep each
+    job's final fields (device tensors) in ``results`` instead of freeing
+    them.  ``inflight``: ticks queued on the device ahead of the host."""
+
+    def __init__(self, hbm_budget: Optional[int] = None, keep_results: bool = False, inflight: int = 2):
+        import torch
+
+        from cutile.runtime import rt
+
+        if hbm_budget is None:
+            free, _total = rt.mem_info()
+            hbm_budget = int(free * 0.9)
+        self.budget = int(hbm_budget)
+        self.keep = keep_results
+        self.inflight = max(1, inflight)
+        self.classes: Dict[tuple, _Class] = {}
+        self.results: Dict[int, dict] = {}
+        self.reserved = 0
+        self.peak_reserved = 0
+        self.stream = torch.cuda.current_stream().cuda_stream
+
+    def _class(self, job: Job) -> _Class:
+        key = (job.kind, job.dtype)
+        if key not in self.classes:
+            self.classes[key] = _Class(*key)
+        return self.classes[key]
+
+    def job_bytes(self, job: Job) -> int:
+        return self._class(job).job_bytes(job.side)
+
+    def run(self, jobs: List[Job], realtime: bool = True, log=None) -> dict:
+        """Serve ``jobs`` (sorted by arrival).  ``realtime``: arrivals follow
+        the wall clock; otherwise every job is present at time 0."""
+        import torch
+
+        from cutile.runtime import rt
+
+        for j in jobs:
+            if self.job_bytes(j) > self.budget:
+                raise ValueError(f"job {j.id} needs {self.job_bytes(j)} bytes, more than the budget {self.budget}")
+        pending = sorted(jobs, key=lambda j: j.arrival)
+        waiting: List[Job] = []
+        active: List[Job] = []
+        ticks: list = []                       # (event, [jobs finishing in that tick])
+        finished: List[Job] = []
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        nticks = 0
+
+        def retire(ev, done_jobs):
+            ev.synchronize()
+            tnow = now()
+            for j in done_jobs:
+                j.finished = tnow
+                cls = self._class(j)
+                if self.keep:
+                    p = j.done % 2
+                    self.results[j.id] = {f: j.bufs[f][p] for f in cls.fields}
+                j.bufs = {}
+                self.reserved -= j.nbytes
+                finished.append(j)
+
+        while pending or waiting or active or ticks:
+            t = now() if realtime else float("inf")
+            while pending and pending[0].arrival <= t:
+                waiting.append(pending.pop(0))
+            # admission (FIFO) by free budget
+            changed = set()
+            while waiting and self.reserved + self.job_bytes(waiting[0]) <= self.budget:
+                j = waiting.pop(0)
+                cls = self._class(j)
+                cls.allocate(j)
+                j.admitted = now()
+                self.reserved += j.nbytes
+                self.peak_reserved = max(self.peak_reserved, self.reserved)
+                cls.jobs.append(j)
+                active.append(j)
+                changed.add(cls)
+            if not active:
+                if ticks:
+                    retire(*ticks.pop(0))
+                    continue
+                if pending and realtime:
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
+                continue
+            for cls in changed:
+                cls.tick = 0
+                cls.rebuild()
+            # one tick: every class advances its jobs by one step
+            for cls in self.classes.values():
+                if cls.jobs:
+                    cls.step(self.stream)
+            ev = rt.Event()
+            ev.record(self.stream)
+            nticks += 1
+            done_now = [j for j in active if j.done >= j.steps]
+            if done_now:
+                active = [j for j in active if j.done < j.steps]
+                for cls in {self._class(j) for j in done_now}:
+                    cls.jobs = [j for j in cls.jobs if j.done < j.steps]
+                    cls.tick = 0
+                    cls.rebuild()
+            ticks.append((ev, done_now))
+            while len(ticks) > self.inflight:
+                retire(*ticks.pop(0))
+            if log and nticks % 500 == 0:
+                log(f"serving: t={now():.1f}s ticks={nticks} active={len(active)} waiting={len(waiting)} "
+                    f"done={len(finished)} reserved={self.reserved / 2**30:.2f} GiB")
+        torch.cuda.synchronize()
+        wall = now()
+        lat = np.array([(j.finished - j.arrival) * 1e3 for j in finished]) if finished else np.zeros(1)
+        updates = sum(j.cells * j.steps for j in finished)
+        return {
+            "jobs": len(finished), "ticks": nticks, "wall_s": wall,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+            "gcell_updates_per_s": updates / wall / 1e9, "cell_updates": updates,
+            "peak_reserved_gib": self.peak_reserved / 2**30, "budget_gib": self.budget / 2**30,
+        }
+
+
+def main(argv=None) -> int:
+    """``python -m cutile.serving [--rate R] [--seconds S] [--seed N]``: the
+    serving configuration (Poisson arrivals, mixed jobs) on the current
+    device; prints one JSON line of latency and throughput figures."""
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--seconds", type=float, default=60.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--budget-gib", type=float, default=None)
+    args = ap.parse_args(argv)
+    jobs = generate_jobs(args.rate, args.seconds, args.seed)
+    srv = StencilServer(hbm_budget=None if args.budget_gib is None else int(args.budget_gib * 2**30))
+    warm = generate_jobs(50.0, 0.2, args.seed + 1)   # build every class's kernels before the clock starts
+    for j in warm:
+        j.steps = 2
+    srv.run(warm, realtime=False)
+    stats = srv.run(jobs, realtime=True,
+                    log=lambda m: print(f"[serving] {m}", file=sys.stderr, flush=True))
+    stats.update(rate=args.rate, seconds=args.seconds, seed=args.seed, offered_jobs=len(jobs))
+    print(json.dumps(stats), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
