@@ -609,6 +609,10 @@ int st_event_destroy(void* ev) {
   HIPCHK(hipEventDestroy((hipEvent_t)ev));
   return ST_OK;
 }
+int st_event_sync(void* ev) {
+  HIPCHK(hipEventSynchronize((hipEvent_t)ev));
+  return ST_OK;
+}
 int st_stream_sync(void* stream) {
   HIPCHK(hipStreamSynchronize((hipStream_t)stream));
   return ST_OK;

@@ -26,7 +26,7 @@ EXPORTS = (
     "st_compile_only", "st_free", "st_compile", "st_load_code", "st_unload", "st_get_function",
     "st_launch", "st_function_resources", "st_fill_halo", "st_fill_halo_batched", "st_copy", "st_diff_norms",
     "st_bricks",
-    "st_event_create", "st_event_record", "st_event_elapsed_ms", "st_event_destroy",
+    "st_event_create", "st_event_record", "st_event_elapsed_ms", "st_event_destroy", "st_event_sync",
     "st_stream_sync", "st_device_sync", "st_graph_begin", "st_graph_end", "st_graph_launch",
     "st_graph_destroy",
 )
@@ -93,6 +93,7 @@ def _declare(lib):
         "st_event_create": ([ctypes.POINTER(vp)], ctypes.c_int),
         "st_event_record": ([vp, vp], ctypes.c_int),
         "st_event_elapsed_ms": ([vp, vp, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
+        "st_event_sync": ([vp], ctypes.c_int),
         "st_event_destroy": ([vp], ctypes.c_int),
         "st_stream_sync": ([vp], ctypes.c_int),
         "st_device_sync": ([], ctypes.c_int),
@@ -257,6 +258,9 @@ class Event:
 
     def record(self, stream=None):
         check(lib().st_event_record(self.h, stream), "st_event_record")
+
+    def synchronize(self) -> None:
+        check(lib().st_event_sync(self.h), "st_event_sync")
 
     def elapsed_ms(self, end: "Event") -> float:
         ms = ctypes.c_float(0)

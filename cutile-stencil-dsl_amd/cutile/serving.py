@@ -112,7 +112,7 @@ class _Class:
         probe = BatchedStencil(fn)
         vz = probe.vz
         if kind == "diffusion3d":
-            buckets = [Bucket("rows", 4, 32 * vz, 16, 96), Bucket("rows", 8, 64 * vz, 32, None)]
+            buckets = [Bucket("rows", 8, 32 * vz, 16, 96), Bucket("rows", 8, 64 * vz, 32, None)]
         else:
             buckets = None                       # the 2-D plane/rows buckets by row length
         self.bs = BatchedStencil(fn, buckets=buckets)

@@ -156,6 +156,7 @@ int st_event_create(void** ev);
 int st_event_record(void* ev, void* stream);
 int st_event_elapsed_ms(void* start, void* stop, float* ms);
 int st_event_destroy(void* ev);
+int st_event_sync(void* ev);         /* wait until the event has completed */
 int st_stream_sync(void* stream);
 int st_device_sync(void);
 
