@@ -274,7 +274,8 @@ def gray_scott_c3(n=4096, steps=1000):
             ms = _timed(side.cuda_stream, lambda: [g.launch() for _ in range(99)], rt)
             side.synchronize()
             h = r.halo_widths
-            early[name] = {t: f[t][h[0]:h[0] + n, h[1]:h[1] + n].to(torch.float64) for t in ("u", "v")}
+            # copy=True: a float64 slice would otherwise alias the live buffer
+            early[name] = {t: f[t][h[0]:h[0] + n, h[1]:h[1] + n].to(torch.float64, copy=True) for t in ("u", "v")}
             ms += _timed(side.cuda_stream, lambda: [g.launch() for _ in range((steps - 200) // 2)], rt)
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()

@@ -47,6 +47,16 @@ DEFAULT_BUCKETS_2D = (
 )
 
 
+def upload(a: np.ndarray):
+    """Device copy of a small host table, queued on the current stream
+    without blocking the host: staged through pinned memory, which the
+    caller keeps alive (returned second) until the stream has used it."""
+    import torch
+
+    host = torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
+    return host.to("cuda", non_blocking=True), host
+
+
 @dataclass
 class BatchPlan:
     table: object                 # device uint8 tensor holding SweepArgs[]
@@ -220,9 +230,10 @@ class BatchedStencil:
             blk[:, 3] = ch.ravel() * 8 + k
             items.append(blk)
         blob = bytes(table)
-        dtab = torch.frombuffer(bytearray(blob), dtype=torch.uint8).to("cuda")
         it = np.concatenate(items) if items else np.zeros((1, 4), dtype=np.int32)
-        ditems = torch.from_numpy(np.ascontiguousarray(it)).to("cuda")
+        dtab, htab = upload(np.frombuffer(blob, dtype=np.uint8))
+        ditems, hitems = upload(np.ascontiguousarray(it))
+        keep += [htab, hitems]
         nitems = int(sum(len(b) for b in items))
         grid = max(1, min(nitems, self.occupancy * cu_count()))
         return BatchPlan(dtab, ditems, nitems, grid, cells, len(instances), keep)

@@ -58,6 +58,7 @@ class Job:
     admitted: Optional[float] = None
     finished: Optional[float] = None
     nbytes: int = 0
+    keep: list = field(default_factory=list, repr=False)   # staging the device still reads
 
     @property
     def ndim(self) -> int:
@@ -162,9 +163,8 @@ class _Class:
             self.fills.append(self._fill_table(flip) if (inst and self.periodic) else None)
 
     def _fill_table(self, flip: int):
-        import torch
-
         from cutile.runtime import rt
+        from cutile.runtime.batched import upload
         from cutile.runtime.kernel import dtype_code
 
         descs = (rt.FillDesc * (len(self.jobs) * len(self.fields)))()
@@ -185,8 +185,8 @@ class _Class:
                     slab = 2 * self.halo[a] * int(np.prod([logical[b] for b in range(self.ndim) if b != a]))
                     max_slab = max(max_slab, slab)
                 k += 1
-        blob = torch.frombuffer(bytearray(bytes(descs)), dtype=torch.uint8).to("cuda")
-        return blob, k, max_slab
+        blob, host = upload(np.frombuffer(bytes(descs), dtype=np.uint8))
+        return blob, k, max_slab, host
 
     def step(self, stream) -> None:
         """Advance every active job of the class by one step."""
@@ -198,7 +198,7 @@ class _Class:
             return
         self.bs.launch(plan, stream)
         if self.fills[flip] is not None:
-            blob, count, max_slab = self.fills[flip]
+            blob, count, max_slab, _host = self.fills[flip]
             rt.fill_halo_batched(blob.data_ptr(), count, self.ndim, max_slab, stream)
         self.tick += 1
         for j in self.jobs:
@@ -242,16 +242,15 @@ def init_fields(job: Job, cls: _Class, bufs: dict) -> None:
         v += 0.01 * torch.rand((s, s), generator=g, device="cuda", dtype=torch.float64)
         bufs["u"][0][sl] = u.to(bufs["u"][0].dtype)
         bufs["v"][0][sl] = v.to(bufs["v"][0].dtype)
-        for f in ("u", "v"):
-            _fill_padded(bufs[f][0], s, h)
+        job.keep = [_fill_padded(bufs[f][0], s, h) for f in ("u", "v")]
 
 
-def _fill_padded(t, side: int, halo) -> None:
+def _fill_padded(t, side: int, halo):
     """Periodic fill of an array whose rows are padded past the logical
-    width (one batched-fill descriptor with a row pitch)."""
-    import torch
-
+    width (one batched-fill descriptor with a row pitch); returns the
+    descriptor buffers, which must live until the stream has run the fill."""
     from cutile.runtime import rt
+    from cutile.runtime.batched import upload
     from cutile.runtime.kernel import default_stream, dtype_code
 
     nd = t.dim()
@@ -264,9 +263,9 @@ def _fill_padded(t, side: int, halo) -> None:
         d.kinds[2 * a] = d.kinds[2 * a + 1] = rt.BC_CODES["periodic"]
         max_slab = max(max_slab, 2 * halo[a] * int(np.prod([logical[b] for b in range(nd) if b != a])))
     d.ndim, d.dtype, d.pitch = nd, dtype_code(t), t.shape[-1]
-    blob = torch.frombuffer(bytearray(bytes(d)), dtype=torch.uint8).to("cuda")
+    blob, host = upload(np.frombuffer(bytes(d), dtype=np.uint8))
     rt.fill_halo_batched(blob.data_ptr(), 1, nd, max_slab, default_stream())
-    torch.cuda.current_stream().synchronize()    # the descriptor blob is freed on return
+    return blob, host
 
 
 # ---------------------------------------------------------------------------
@@ -336,6 +335,7 @@ class StencilServer:
                     p = j.done % 2
                     self.results[j.id] = {f: j.bufs[f][p] for f in cls.fields}
                 j.bufs = {}
+                j.keep = []
                 self.reserved -= j.nbytes
                 finished.append(j)
 
