@@ -158,6 +158,82 @@ class BatchedStencil:
                 return k
         return len(self.buckets) - 1
 
+    def record(self, ins: Sequence, outs: Sequence, halo: Sequence[int], inner: Sequence[int]):
+        """Launch record of one instance: ``(args bytes, work items with
+        instance index 0, interior cells)`` or ``None`` for an empty
+        interior.  Records can be cached and assembled into plans cheaply
+        (the serving loop rebuilds its work-lists from cached records)."""
+        from cutile.runtime.kernel import SweepArgs
+
+        nd = self.ndim
+        if len(ins) != self.num_inputs or len(outs) != self.num_outputs:
+            raise ValueError(f"expected {self.num_inputs} inputs and {self.num_outputs} outputs")
+        shape = tuple(ins[0].shape)
+        inner = tuple(inner)
+        if any(n + 2 * h > s for n, h, s in zip(inner, halo, shape)):
+            raise ValueError(f"interior {inner} + halo exceeds the array {shape}")
+        if min(inner) <= 0:
+            return None
+        k = self.bucket_of(inner[-1])
+        b = self.buckets[k]
+        if nd == 2:
+            if b.mapping == "plane":
+                s3, h3, n3 = (1, shape[0], shape[1]), (0, halo[0], halo[1]), (1, inner[0], inner[1])
+            else:
+                s3, h3, n3 = (shape[0], 1, shape[1]), (halo[0], 0, halo[1]), (inner[0], 1, inner[1])
+        elif nd == 1:
+            s3, h3, n3 = (1, 1, shape[0]), (0, 0, halo[0]), (1, 1, inner[0])
+        else:
+            s3, h3, n3 = shape, tuple(halo), inner
+        if h3[2] % self.vz or s3[2] % self.vz:
+            raise ValueError(f"contiguous extent/halo not a multiple of {self.vz}")
+        if s3[1] * s3[2] * ins[0].element_size() >= 1 << 31:
+            raise ValueError("a plane exceeds the 2 GiB range of one buffer resource")
+        a = SweepArgs()
+        for f, t in enumerate(ins):
+            a.inp[f] = t.data_ptr()
+        for m, t in enumerate(outs):
+            a.out[m] = t.data_ptr()
+        for t in list(ins) + list(outs):
+            if t.data_ptr() % 16:
+                raise ValueError("arrays must be 16-byte aligned")
+        a.s0, a.s1, a.s2 = s3
+        a.h0, a.h1, a.h2 = h3
+        a.n0, a.n1, a.n2 = n3
+        a.ntz = -(-n3[2] // b.tz)
+        a.nty = -(-n3[1] // b.ty)
+        a.xch = max(1, min(b.xch, n3[0]))
+        a.nch = -(-n3[0] // a.xch)
+        a.nxcd = 1
+        ch, ty, tz = np.meshgrid(np.arange(a.nch), np.arange(a.nty), np.arange(a.ntz), indexing="ij")
+        blk = np.empty((ch.size, 4), dtype=np.int32)
+        blk[:, 0] = 0
+        blk[:, 1] = tz.ravel()
+        blk[:, 2] = ty.ravel()
+        blk[:, 3] = ch.ravel() * 8 + k
+        return bytes(a), blk, math.prod(inner)
+
+    def assemble(self, records: Sequence, keep: Sequence = ()) -> BatchPlan:
+        """One plan (device table of launch records + work-list) from
+        records of :meth:`record`; ``keep`` holds tensors the plan uses."""
+        from cutile.runtime.kernel import cu_count
+
+        recs = [r for r in records if r is not None]
+        if recs:
+            blob = b"".join(r[0] for r in recs)
+            counts = np.array([len(r[1]) for r in recs])
+            it = np.concatenate([r[1] for r in recs])
+            it[:, 0] = np.repeat(np.arange(len(recs), dtype=np.int32), counts)
+        else:
+            blob = bytes(1)
+            it = np.zeros((1, 4), dtype=np.int32)
+        dtab, htab = upload(np.frombuffer(blob, dtype=np.uint8))
+        ditems, hitems = upload(np.ascontiguousarray(it))
+        nitems = int(sum(len(r[1]) for r in recs))
+        grid = max(1, min(nitems, self.occupancy * cu_count()))
+        return BatchPlan(dtab, ditems, nitems, grid, int(sum(r[2] for r in recs)), len(recs),
+                         list(keep) + [htab, hitems])
+
     def prepare(self, instances: Sequence, halo: Sequence[int], interiors: Optional[Sequence] = None) -> BatchPlan:
         """``instances``: sequence of ``(inputs, outputs)`` tensor lists, all
         with the same leading halo; returns the device tables for one launch.
@@ -165,78 +241,16 @@ class BatchedStencil:
         ``interiors`` gives each instance's interior extents when its arrays
         are padded beyond the symmetric halo on the high side (rows padded to
         a multiple of the 16-byte vector keep every row aligned)."""
-        import torch
-
-        from cutile.runtime.kernel import MAXF, SweepArgs, cu_count, xcd_count, current_arch
-
-        nd = self.ndim
-        table = (SweepArgs * max(1, len(instances)))()
-        items: List[tuple] = []
-        cells = 0
-        keep = []
+        recs, keep = [], []
         for idx, (ins, outs) in enumerate(instances):
-            if len(ins) != self.num_inputs or len(outs) != self.num_outputs:
-                raise ValueError(f"instance {idx}: expected {self.num_inputs} inputs and "
-                                 f"{self.num_outputs} outputs")
-            shape = tuple(ins[0].shape)
             inner = tuple(interiors[idx]) if interiors is not None else \
-                tuple(s - 2 * h for s, h in zip(shape, halo))
-            if any(n + 2 * h > s for n, h, s in zip(inner, halo, shape)):
-                raise ValueError(f"instance {idx}: interior {inner} + halo exceeds the array {shape}")
-            if min(inner) <= 0:
-                continue
-            cells += math.prod(inner)
-            k = self.bucket_of(inner[-1])
-            b = self.buckets[k]
-            if nd == 2:
-                if b.mapping == "plane":
-                    s3, h3 = (1, shape[0], shape[1]), (0, halo[0], halo[1])
-                else:
-                    s3, h3 = (shape[0], 1, shape[1]), (halo[0], 0, halo[1])
-            elif nd == 1:
-                s3, h3 = (1, 1, shape[0]), (0, 0, halo[0])
-            else:
-                s3, h3 = shape, tuple(halo)
-            if nd == 2:
-                n3 = (1, inner[0], inner[1]) if b.mapping == "plane" else (inner[0], 1, inner[1])
-            elif nd == 1:
-                n3 = (1, 1, inner[0])
-            else:
-                n3 = tuple(inner)
-            if h3[2] % self.vz or s3[2] % self.vz:
-                raise ValueError(f"instance {idx}: contiguous extent/halo not a multiple of {self.vz}")
-            a = table[idx]
-            for f, t in enumerate(ins):
-                a.inp[f] = t.data_ptr()
-            for m, t in enumerate(outs):
-                a.out[m] = t.data_ptr()
-            for t in list(ins) + list(outs):
-                if t.data_ptr() % 16:
-                    raise ValueError(f"instance {idx}: arrays must be 16-byte aligned")
+                tuple(s - 2 * h for s, h in zip(ins[0].shape, halo))
+            try:
+                recs.append(self.record(ins, outs, halo, inner))
+            except ValueError as e:
+                raise ValueError(f"instance {idx}: {e}") from None
             keep += list(ins) + list(outs)
-            a.s0, a.s1, a.s2 = s3
-            a.h0, a.h1, a.h2 = h3
-            a.n0, a.n1, a.n2 = n3
-            a.ntz = -(-n3[2] // b.tz)
-            a.nty = -(-n3[1] // b.ty)
-            a.xch = max(1, min(b.xch, n3[0]))
-            a.nch = -(-n3[0] // a.xch)
-            a.nxcd = 1
-            ch, ty, tz = np.meshgrid(np.arange(a.nch), np.arange(a.nty), np.arange(a.ntz), indexing="ij")
-            blk = np.empty((ch.size, 4), dtype=np.int32)
-            blk[:, 0] = idx
-            blk[:, 1] = tz.ravel()
-            blk[:, 2] = ty.ravel()
-            blk[:, 3] = ch.ravel() * 8 + k
-            items.append(blk)
-        blob = bytes(table)
-        it = np.concatenate(items) if items else np.zeros((1, 4), dtype=np.int32)
-        dtab, htab = upload(np.frombuffer(blob, dtype=np.uint8))
-        ditems, hitems = upload(np.ascontiguousarray(it))
-        keep += [htab, hitems]
-        nitems = int(sum(len(b) for b in items))
-        grid = max(1, min(nitems, self.occupancy * cu_count()))
-        return BatchPlan(dtab, ditems, nitems, grid, cells, len(instances), keep)
+        return self.assemble(recs, keep)
 
     def launch(self, plan: BatchPlan, stream=None) -> None:
         from cutile.runtime.kernel import default_stream

@@ -59,6 +59,8 @@ class Job:
     finished: Optional[float] = None
     nbytes: int = 0
     keep: list = field(default_factory=list, repr=False)   # staging the device still reads
+    recs: list = field(default_factory=list, repr=False)   # launch records, both parities
+    fdesc: Optional[list] = field(default=None, repr=False)
 
     @property
     def ndim(self) -> int:
@@ -138,6 +140,8 @@ class _Class:
 
     # -- buffers ----------------------------------------------------------------
     def allocate(self, job: Job) -> None:
+        """Resident buffers, seeded initial fields, and the job's cached
+        launch records and fill descriptors for both buffer parities."""
         import torch
 
         tdt = getattr(torch, self.dtype)
@@ -145,48 +149,55 @@ class _Class:
         job.bufs = {f: [torch.zeros(shp, dtype=tdt, device="cuda") for _ in range(2)] for f in self.fields}
         job.nbytes = self.job_bytes(job.side)
         init_fields(job, self, job.bufs)
+        inner = (job.side,) * self.ndim
+        job.recs = [self.bs.record([job.bufs[f][p] for f in self.fields], [job.bufs[f][1 - p] for f in self.fields],
+                                   self.halo, inner) for p in (0, 1)]
+        job.fdesc = [self._fill_descs(job, 1 - p) for p in (0, 1)] if self.periodic else None
 
     # -- plans -------------------------------------------------------------------
     def rebuild(self) -> None:
         """Work-lists (and periodic fill tables) for the active set at both
-        buffer parities."""
+        buffer parities, assembled from the jobs' cached records."""
+        from cutile.runtime.batched import upload
+
         self.plans, self.fills = [], []
         for flip in (0, 1):
-            inst, interiors = [], []
-            for j in self.jobs:
-                p = (j.done + flip) % 2
-                ins = [j.bufs[f][p] for f in self.fields]
-                outs = [j.bufs[f][1 - p] for f in self.fields]
-                inst.append((ins, outs))
-                interiors.append((j.side,) * self.ndim)
-            self.plans.append(self.bs.prepare(inst, self.halo, interiors) if inst else None)
-            self.fills.append(self._fill_table(flip) if (inst and self.periodic) else None)
+            if not self.jobs:
+                self.plans.append(None)
+                self.fills.append(None)
+                continue
+            par = [(j.done + flip) % 2 for j in self.jobs]
+            self.plans.append(self.bs.assemble([j.recs[p] for j, p in zip(self.jobs, par)]))
+            if self.periodic:
+                blob = b"".join(j.fdesc[p][0] for j, p in zip(self.jobs, par))
+                dev, host = upload(np.frombuffer(blob, dtype=np.uint8))
+                count = len(self.jobs) * len(self.fields)
+                max_slab = max(j.fdesc[p][1] for j, p in zip(self.jobs, par))
+                self.fills.append((dev, count, max_slab, host))
+            else:
+                self.fills.append(None)
 
-    def _fill_table(self, flip: int):
+    def _fill_descs(self, job: Job, which: int) -> tuple:
+        """Periodic fill descriptors (bytes) of the job's buffers ``which``
+        (the outputs of a step from the other parity) and their largest slab."""
         from cutile.runtime import rt
-        from cutile.runtime.batched import upload
         from cutile.runtime.kernel import dtype_code
 
-        descs = (rt.FillDesc * (len(self.jobs) * len(self.fields)))()
-        k, max_slab = 0, 1
-        for j in self.jobs:
-            p = (j.done + flip) % 2
-            for f in self.fields:
-                t = j.bufs[f][1 - p]
-                d = descs[k]
-                d.ptr = t.data_ptr()
-                logical = tuple(j.side + 2 * h for h in self.halo)
-                for a in range(self.ndim):
-                    d.shape[a] = logical[a]
-                    d.halo[a] = self.halo[a]
-                    d.kinds[2 * a] = d.kinds[2 * a + 1] = rt.BC_CODES["periodic"]
-                d.ndim, d.dtype, d.pitch = self.ndim, dtype_code(t), t.shape[-1]
-                for a in range(self.ndim):
-                    slab = 2 * self.halo[a] * int(np.prod([logical[b] for b in range(self.ndim) if b != a]))
-                    max_slab = max(max_slab, slab)
-                k += 1
-        blob, host = upload(np.frombuffer(bytes(descs), dtype=np.uint8))
-        return blob, k, max_slab, host
+        descs = (rt.FillDesc * len(self.fields))()
+        logical = tuple(job.side + 2 * h for h in self.halo)
+        max_slab = 1
+        for k, f in enumerate(self.fields):
+            t = job.bufs[f][which]
+            d = descs[k]
+            d.ptr = t.data_ptr()
+            for a in range(self.ndim):
+                d.shape[a] = logical[a]
+                d.halo[a] = self.halo[a]
+                d.kinds[2 * a] = d.kinds[2 * a + 1] = rt.BC_CODES["periodic"]
+                slab = 2 * self.halo[a] * int(np.prod([logical[b] for b in range(self.ndim) if b != a]))
+                max_slab = max(max_slab, slab)
+            d.ndim, d.dtype, d.pitch = self.ndim, dtype_code(t), t.shape[-1]
+        return bytes(descs), max_slab
 
     def step(self, stream) -> None:
         """Advance every active job of the class by one step."""
@@ -336,6 +347,8 @@ class StencilServer:
                     self.results[j.id] = {f: j.bufs[f][p] for f in cls.fields}
                 j.bufs = {}
                 j.keep = []
+                j.recs = []
+                j.fdesc = None
                 self.reserved -= j.nbytes
                 finished.append(j)
 
