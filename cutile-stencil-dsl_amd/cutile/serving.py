@@ -335,9 +335,12 @@ class StencilServer:
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         nticks = 0
+        host = {"admit": 0.0, "rebuild": 0.0, "launch": 0.0, "wait": 0.0, "idle": 0.0}
 
         def retire(ev, done_jobs):
+            tw = time.perf_counter()
             ev.synchronize()
+            host["wait"] += time.perf_counter() - tw
             tnow = now()
             for j in done_jobs:
                 j.finished = tnow
@@ -358,6 +361,7 @@ class StencilServer:
                 waiting.append(pending.pop(0))
             # admission (FIFO) by free budget
             changed = set()
+            ta = time.perf_counter()
             while waiting and self.reserved + self.job_bytes(waiting[0]) <= self.budget:
                 j = waiting.pop(0)
                 cls = self._class(j)
@@ -368,30 +372,39 @@ class StencilServer:
                 cls.jobs.append(j)
                 active.append(j)
                 changed.add(cls)
+            host["admit"] += time.perf_counter() - ta
             if not active:
                 if ticks:
                     retire(*ticks.pop(0))
                     continue
                 if pending and realtime:
+                    ti = time.perf_counter()
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
+                    host["idle"] += time.perf_counter() - ti
                 continue
+            tr = time.perf_counter()
             for cls in changed:
                 cls.tick = 0
                 cls.rebuild()
+            host["rebuild"] += time.perf_counter() - tr
             # one tick: every class advances its jobs by one step
+            tl = time.perf_counter()
             for cls in self.classes.values():
                 if cls.jobs:
                     cls.step(self.stream)
             ev = rt.Event()
             ev.record(self.stream)
+            host["launch"] += time.perf_counter() - tl
             nticks += 1
             done_now = [j for j in active if j.done >= j.steps]
             if done_now:
+                tr = time.perf_counter()
                 active = [j for j in active if j.done < j.steps]
                 for cls in {self._class(j) for j in done_now}:
                     cls.jobs = [j for j in cls.jobs if j.done < j.steps]
                     cls.tick = 0
                     cls.rebuild()
+                host["rebuild"] += time.perf_counter() - tr
             ticks.append((ev, done_now))
             while len(ticks) > self.inflight:
                 retire(*ticks.pop(0))
@@ -408,6 +421,9 @@ class StencilServer:
             "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
             "gcell_updates_per_s": updates / wall / 1e9, "cell_updates": updates,
             "peak_reserved_gib": self.peak_reserved / 2**30, "budget_gib": self.budget / 2**30,
+            # host seconds: admitting (allocation + on-device init), rebuilding work-lists,
+            # enqueueing ticks, waiting for the device, idle before arrivals
+            "host_s": {k: round(v, 3) for k, v in host.items()},
         }
 
 
