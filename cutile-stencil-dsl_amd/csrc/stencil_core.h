@@ -191,28 +191,28 @@ ST_HD Rsrc make_rsrc(const void* base, int bytes) {
 template <int B> struct BufIO;
 template <> struct BufIO<16> {
   typedef unsigned T __attribute__((ext_vector_type(4)));
-  static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0); }
+  template <int AUX> static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX); }
   template <int AUX> static ST_HD void st(T v, Rsrc r, int off) { __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, AUX); }
 };
 template <> struct BufIO<8> {
   typedef unsigned T __attribute__((ext_vector_type(2)));
-  static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0); }
+  template <int AUX> static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX); }
   template <int AUX> static ST_HD void st(T v, Rsrc r, int off) { __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, AUX); }
 };
 template <> struct BufIO<4> {
   typedef unsigned T;
-  static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0); }
+  template <int AUX> static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, AUX); }
   template <int AUX> static ST_HD void st(T v, Rsrc r, int off) { __builtin_amdgcn_raw_buffer_store_b32(v, r, off, 0, AUX); }
 };
 template <> struct BufIO<2> {
   typedef unsigned short T;
-  static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0); }
+  template <int AUX> static ST_HD T ld(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, AUX); }
   template <int AUX> static ST_HD void st(T v, Rsrc r, int off) { __builtin_amdgcn_raw_buffer_store_b16(v, r, off, 0, AUX); }
 };
 
-template <class S, int V> ST_HD Vec<S, V> bload(Rsrc r, int off) {
+template <class S, int V, int AUX = 0> ST_HD Vec<S, V> bload(Rsrc r, int off) {
   using IO = BufIO<(int)sizeof(S) * V>;
-  return __builtin_bit_cast(Vec<S, V>, IO::ld(r, off));
+  return __builtin_bit_cast(Vec<S, V>, IO::template ld<AUX>(r, off));
 }
 template <class S, int V, int AUX> ST_HD void bstore(const Vec<S, V>& x, Rsrc r, int off) {
   using IO = BufIO<(int)sizeof(S) * V>;
@@ -277,7 +277,8 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       const int y = y0 + ty + k * NTY;
       const int yq = w1 ? wrapi(y, a.n1) : y;
       const bool ok = live && zload && a.h1 + yq < s1 && ty + k * NTY < TY;
-      const Vec<S, VZ> w = bload<S, VZ>(r, ok ? ((a.h1 + yq) * s2 + a.h2 + zq) * ES : BUF_OOB);
+      // a queue plane is read once per chunk: optionally non-temporal (P::NTL)
+      const Vec<S, VZ> w = bload<S, VZ, P::NTL ? BUF_NT : 0>(r, ok ? ((a.h1 + yq) * s2 + a.h2 + zq) * ES : BUF_OOB);
 #pragma unroll
       for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
     }

@@ -100,7 +100,8 @@ def temporal_tile(ndim: int, T: int, r3: tuple, compute: str, budget: int = 160 
 
 
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,
-                nontemporal: bool = True, prefetch: int = 1, partial: bool = False) -> str:
+                nontemporal: bool = True, prefetch: int = 1, partial: bool = False,
+                ntload: bool = False) -> str:
     """Traits struct of one kernel variant.  ``partial``: rows may end in a
     partial vector (batched instances whose interior is not a multiple of
     the vector width)."""
@@ -122,7 +123,7 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int BRZ[{max(1, len(bufs))}] = {_arr([b[3] for b in bufs])};",
         f"  static constexpr int RX = {r3[0]}, RY = {r3[1]}, RZ = {r3[2]};",
         f"  static constexpr int NTS = {1 if nontemporal else 0}, PD = {int(prefetch)}, "
-        f"PARTIAL = {1 if partial else 0};",
+        f"PARTIAL = {1 if partial else 0}, NTL = {1 if ntload else 0};",
     ]
     upd = meta.update
     ne = meta.num_exprs
@@ -155,7 +156,8 @@ def temporal_symbol(meta: StencilMeta, tag: str, T: int) -> str:
 
 
 def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage: str,
-                    compute: str, temporal=None, nontemporal: bool = True, prefetch: int = 1) -> str:
+                    compute: str, temporal=None, nontemporal: bool = True, prefetch: int = 1,
+                    ntload: bool = False) -> str:
     """Full HIP translation unit with one sweep kernel per variant."""
     out = [
         f"// {meta.name}: generated by the HIP emitter for gfx950",
@@ -165,7 +167,7 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
     ]
     for cfg in variants:
         tag = cfg.tag
-        out.append(emit_traits(meta, cfg, storage, compute, tag, nontemporal, prefetch))
+        out.append(emit_traits(meta, cfg, storage, compute, tag, nontemporal, prefetch, ntload=ntload))
         out.append(
             f'extern "C" __global__ void __launch_bounds__({cfg.nt}) '
             f"{kernel_symbol(meta, tag)}(st::SweepArgs a) {{ st::sweep_entry<P_{tag}>(a); }}"

@@ -123,7 +123,8 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
         temporal = (KernelConfig(ty=ty, tz=tz, vz=1, nt=256), int(temporal_steps))
     src = emit_hip_source(meta, [vec, scal], storage_t, compute_t, temporal,
                           nontemporal=bool((kernel_hints or {}).get("nontemporal", 1)),
-                          prefetch=int((kernel_hints or {}).get("prefetch", 1)))
+                          prefetch=int((kernel_hints or {}).get("prefetch", 1)),
+                          ntload=bool((kernel_hints or {}).get("ntload", 0)))
     attr = _attrs(kernel_hints)
     if attr:
         src = src.replace('extern "C" __global__ void ', f'extern "C" __global__ void {attr}')
