@@ -349,6 +349,8 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
                                            kernel_hints=cfg.kernel_hints, storage=storage, update=update,
                                            dt=dt, targets=targets)
 
+        if kernel_hints:   # emitter options the caller asked for apply to every candidate
+            candidates = [(t, dict(h, **kernel_hints)) for t, h in candidates]
         tiles, hints, resources = _choose_config(
             _budgeted_configs(tiling, candidates), _lower, kernel_name=f"{name}_kernel",
             ndim=len(halo), num_inputs=len(_input_names(ir)), halo_widths=halo, dtype=dtype,
