@@ -427,26 +427,24 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
         }
       }
       const int off = ((a.h1 + y) * s2 + a.h2 + zc) * ES;
-      if (!P::PARTIAL || zc + VZ <= a.n2 || !zin) {
-        // whole vectors: one unconditional store per output
+      // one unconditional vector store per output.  With PARTIAL (rows whose
+      // interior is not a multiple of VZ: batched instances with padded
+      // rows) the lane holding a row's ragged end first reads the output's
+      // cells past the interior and writes them back unchanged, so its
+      // store stays a whole vector and no branch guards a memory access
+      const bool ragged = P::PARTIAL && valid && zc + VZ > a.n2;
 #pragma unroll
-        for (int m = 0; m < NOUT; ++m) {
-          Vec<S, VZ> w;
+      for (int m = 0; m < NOUT; ++m) {
+        Vec<S, VZ> w;
+        if constexpr (P::PARTIAL) {
+          const Vec<S, VZ> keep = bload<S, VZ>(ro[m], ragged ? off : BUF_OOB);
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) w.v[v] = zc + v < a.n2 ? Cvt<S, C>::st(res[m][v]) : keep.v[v];
+        } else {
 #pragma unroll
           for (int v = 0; v < VZ; ++v) w.v[v] = Cvt<S, C>::st(res[m][v]);
-          bstore<S, VZ, P::NTS ? BUF_NT : 0>(w, ro[m], valid ? off : BUF_OOB);
         }
-      } else {
-        // the ragged end of a row whose interior is not a multiple of VZ
-        // (batched instances with padded rows)
-#pragma unroll
-        for (int m = 0; m < NOUT; ++m)
-#pragma unroll
-          for (int v = 0; v < VZ; ++v) {
-            Vec<S, 1> e;
-            e.v[0] = Cvt<S, C>::st(res[m][v]);
-            bstore<S, 1, 0>(e, ro[m], valid && zc + v < a.n2 ? off + v * ES : BUF_OOB);
-          }
+        bstore<S, VZ, P::NTS ? BUF_NT : 0>(w, ro[m], valid ? off : BUF_OOB);
       }
     }
   };
