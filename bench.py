@@ -487,6 +487,8 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--core", action="store_true",
+                    help="extras limited to the other 3-D sweeps and the copy reference (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py drives one GPU: this build runs every workload on a single MI355X "
@@ -519,7 +521,7 @@ def main():
                                        args.warmup, "f32", spec)
         extra["copy_ceiling"] = copy_ceiling(head["alg_bytes_per_launch"] // 2)
         log(f"copy ceiling: {extra['copy_ceiling']['gbs']:.0f} GB/s")
-        for key, fn in (("temporal_depths_c2", lambda: temporal_depths(interior, n)),):
+        for key, fn in (() if args.core else (("temporal_depths_c2", lambda: temporal_depths(interior, n)),)):
             try:
                 extra[key] = fn()
             except Exception as e:  # an extra must not hide the headline line
@@ -527,7 +529,7 @@ def main():
                 log(f"{key} failed: {e}")
     del interior
     torch.cuda.empty_cache()
-    if not args.no_extra:
+    if not args.no_extra and not args.core:
         for key, fn in (("heat2d_c1", heat2d_c1), ("gray_scott_c3", gray_scott_c3),
                         ("batched_wave_c4", batched_wave_c4), ("serving_c5", serving_c5)):
             try:
