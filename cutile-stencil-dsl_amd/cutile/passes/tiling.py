@@ -17,7 +17,7 @@ ranked list of ``(tile_sizes, kernel_hints)``) and the ``tile_sizes`` /
 * the first candidate per tier and element size is the shape measured
   fastest on MI355X at 512^3 among variants that neither spill nor exceed
   256 VGPRs (7-point float64: 16 x 256 rows x cols on 512 threads;
-  25-point float64: 16 x 128 on 512 threads; 25-point float32: 16 x 256 on
+  25-point float64: 32 x 64 on 512 threads; 25-point float32: 32 x 128 on
   512 threads); taller tiles cut the halo rows re-read through L2, which is
   what bounds the wide stencils.  The rest of the ladder halves the rows,
   then adds an ``occupancy`` hint (``amdgpu_waves_per_eu``, a VGPR budget of
@@ -85,8 +85,8 @@ class TilingPass:
     # and float32 entries are the fastest non-spilling shapes measured on
     # MI355X at 512^3, the bfloat16 ones keep the float32 register footprint
     _TILES_3D = {
-        (8, "narrow"): (16, 256, 512), (8, "wide"): (16, 128, 512), (8, "very_wide"): (4, 128, 256),
-        (4, "narrow"): (16, 256, 512), (4, "wide"): (16, 256, 512), (4, "very_wide"): (4, 256, 256),
+        (8, "narrow"): (16, 256, 512), (8, "wide"): (32, 64, 512), (8, "very_wide"): (4, 128, 256),
+        (4, "narrow"): (16, 256, 512), (4, "wide"): (32, 128, 512), (4, "very_wide"): (4, 256, 256),
         (2, "narrow"): (16, 512, 512), (2, "wide"): (8, 512, 512), (2, "very_wide"): (4, 512, 256),
     }
 

@@ -124,9 +124,9 @@ class TestPasses:
         assert t.candidates(2, 4) == [((64, 512), {})]
         assert t.candidates(1, 3) == [((512,),  {})]
         assert TilingPass(dtype_bytes=4).candidates(3, 7) == [((256, 16, 256), {"threads": 512})]
-        assert TilingPass(dtype_bytes=4).candidates(3, 25)[0] == ((256, 16, 256), {"threads": 512})
+        assert TilingPass(dtype_bytes=4).candidates(3, 25)[0] == ((256, 32, 128), {"threads": 512})
         wide = t.candidates(3, 25)
-        assert wide[:2] == [((256, 16, 128), {"threads": 512}), ((256, 8, 128), {"threads": 512})]
+        assert wide[:2] == [((256, 32, 64), {"threads": 512}), ((256, 16, 64), {"threads": 512})]
         assert wide[2][1] == {"threads": 512, "occupancy": 4}
         very = t.candidates(3, 125)
         assert very[0] == ((256, 4, 128), {}) and very[1][1] == {"occupancy": 4}
