@@ -10,9 +10,11 @@ this is the batching runtime the serving loop and the batched benchmark use.
   The kernel contains one sweep instantiation per bucket and dispatches on
   the work item's bucket.
 * The host builds, once per set of instances, a device table of per-instance
-  launch arguments (``st::SweepArgs``) and a work-list of
-  ``(instance, tile_z, tile_y, chunk|bucket)`` items; a fixed grid of
-  ``occupancy x CUs`` workgroups strides through the list, so the launch
+  launch arguments (``st::SweepArgs``), the first work item of every
+  instance and its tile bucket; item i belongs to the instance found by a
+  binary search over the first items and is decoded into (tile z, tile y,
+  chunk) from that instance's tile counts.  A fixed grid of
+  ``occupancy x CUs`` workgroups strides through the items, so the launch
   shape does not depend on the batch and a captured hipGraph stays valid.
 """
 
@@ -53,14 +55,15 @@ def upload(a: np.ndarray):
     caller keeps alive (returned second) until the stream has used it."""
     import torch
 
-    host = torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
+    host = torch.from_numpy(np.array(a, copy=True)).pin_memory()
     return host.to("cuda", non_blocking=True), host
 
 
 @dataclass
 class BatchPlan:
     table: object                 # device uint8 tensor holding SweepArgs[]
-    items: object                 # device int32 tensor (n, 4)
+    first: object                 # device int32: first work item of every instance (+ total)
+    bucket: object                # device int32: tile bucket of every instance
     nitems: int
     grid: int
     cells: int
@@ -119,18 +122,27 @@ class BatchedStencil:
             parts.append(emit_traits(meta, cfg, self.storage, self.compute, f"b{k}", partial=True))
             cases.append(f"      case {k}: st::sweep_tile<P_b{k}>(a, tz, ty, ch); break;")
         self.symbol = f"{self.name}_batched"
-        parts.append("struct BatchItem { int inst, tz, ty, cb; };")
+        # The work-list is implicit: instance k owns items first[k] ..
+        # first[k+1]-1, decoded into (tile z, tile y, chunk) from its own
+        # tile counts; a workgroup finds the instance of item i by binary
+        # search over first[] (uniform, so scalar loads).  The host uploads
+        # only per-instance records, so the list is cheap to rebuild.
         parts.append(
             f'extern "C" __global__ void __launch_bounds__(256) {self.symbol}('
-            "const st::SweepArgs* __restrict__ insts, const BatchItem* __restrict__ items, int nitems) {\n"
+            "const st::SweepArgs* __restrict__ insts, const int* __restrict__ first, "
+            "const int* __restrict__ bucket, int ninst, int nitems) {\n"
             "  for (int i = blockIdx.x; i < nitems; i += gridDim.x) {\n"
-            "    const int inst = __builtin_amdgcn_readfirstlane(items[i].inst);\n"
-            "    const int tz = __builtin_amdgcn_readfirstlane(items[i].tz);\n"
-            "    const int ty = __builtin_amdgcn_readfirstlane(items[i].ty);\n"
-            "    const int cb = __builtin_amdgcn_readfirstlane(items[i].cb);\n"
-            "    const int ch = cb >> 3;\n"
+            "    int lo = 0, hi = ninst - 1;\n"
+            "    while (lo < hi) {\n"
+            "      const int mid = (lo + hi + 1) >> 1;\n"
+            "      if (first[mid] <= i) lo = mid; else hi = mid - 1;\n"
+            "    }\n"
+            "    const int inst = __builtin_amdgcn_readfirstlane(lo);\n"
             "    const st::SweepArgs& a = insts[inst];\n"
-            "    switch (cb & 7) {\n" + "\n".join(cases) + "\n      default: break;\n    }\n"
+            "    const int local = i - first[inst];\n"
+            "    const int tz = local % a.ntz, rest = local / a.ntz;\n"
+            "    const int ty = rest % a.nty, ch = rest / a.nty;\n"
+            "    switch (bucket[inst]) {\n" + "\n".join(cases) + "\n      default: break;\n    }\n"
             "  }\n}"
         )
         self.source = "\n".join(parts)
@@ -159,10 +171,10 @@ class BatchedStencil:
         return len(self.buckets) - 1
 
     def record(self, ins: Sequence, outs: Sequence, halo: Sequence[int], inner: Sequence[int]):
-        """Launch record of one instance: ``(args bytes, work items with
-        instance index 0, interior cells)`` or ``None`` for an empty
-        interior.  Records can be cached and assembled into plans cheaply
-        (the serving loop rebuilds its work-lists from cached records)."""
+        """Launch record of one instance: ``(args bytes, work items, bucket,
+        interior cells)`` or ``None`` for an empty interior.  Records can be
+        cached and assembled into plans cheaply (the serving loop rebuilds
+        its work-lists from cached records)."""
         from cutile.runtime.kernel import SweepArgs
 
         nd = self.ndim
@@ -205,34 +217,29 @@ class BatchedStencil:
         a.xch = max(1, min(b.xch, n3[0]))
         a.nch = -(-n3[0] // a.xch)
         a.nxcd = 1
-        ch, ty, tz = np.meshgrid(np.arange(a.nch), np.arange(a.nty), np.arange(a.ntz), indexing="ij")
-        blk = np.empty((ch.size, 4), dtype=np.int32)
-        blk[:, 0] = 0
-        blk[:, 1] = tz.ravel()
-        blk[:, 2] = ty.ravel()
-        blk[:, 3] = ch.ravel() * 8 + k
-        return bytes(a), blk, math.prod(inner)
+        return bytes(a), a.nch * a.nty * a.ntz, k, math.prod(inner)
 
     def assemble(self, records: Sequence, keep: Sequence = ()) -> BatchPlan:
-        """One plan (device table of launch records + work-list) from
-        records of :meth:`record`; ``keep`` holds tensors the plan uses."""
+        """One plan (device tables of launch records, first item of every
+        instance, bucket of every instance) from records of :meth:`record`;
+        ``keep`` holds tensors the plan uses."""
         from cutile.runtime.kernel import cu_count
 
         recs = [r for r in records if r is not None]
         if recs:
             blob = b"".join(r[0] for r in recs)
-            counts = np.array([len(r[1]) for r in recs])
-            it = np.concatenate([r[1] for r in recs])
-            it[:, 0] = np.repeat(np.arange(len(recs), dtype=np.int32), counts)
+            first = np.zeros(len(recs) + 1, dtype=np.int32)
+            np.cumsum([r[1] for r in recs], out=first[1:])
+            bucket = np.array([r[2] for r in recs], dtype=np.int32)
         else:
-            blob = bytes(1)
-            it = np.zeros((1, 4), dtype=np.int32)
+            blob, first, bucket = bytes(1), np.zeros(1, dtype=np.int32), np.zeros(1, dtype=np.int32)
         dtab, htab = upload(np.frombuffer(blob, dtype=np.uint8))
-        ditems, hitems = upload(np.ascontiguousarray(it))
-        nitems = int(sum(len(r[1]) for r in recs))
+        dfirst, hfirst = upload(first)
+        dbucket, hbucket = upload(bucket)
+        nitems = int(first[-1])
         grid = max(1, min(nitems, self.occupancy * cu_count()))
-        return BatchPlan(dtab, ditems, nitems, grid, int(sum(r[2] for r in recs)), len(recs),
-                         list(keep) + [htab, hitems])
+        return BatchPlan(dtab, dfirst, dbucket, nitems, grid, int(sum(r[3] for r in recs)), len(recs),
+                         list(keep) + [htab, hfirst, hbucket])
 
     def prepare(self, instances: Sequence, halo: Sequence[int], interiors: Optional[Sequence] = None) -> BatchPlan:
         """``instances``: sequence of ``(inputs, outputs)`` tensor lists, all
@@ -259,9 +266,11 @@ class BatchedStencil:
             return
 
         class _Args(ctypes.Structure):
-            _fields_ = [("insts", ctypes.c_void_p), ("items", ctypes.c_void_p), ("nitems", ctypes.c_int)]
+            _fields_ = [("insts", ctypes.c_void_p), ("first", ctypes.c_void_p), ("bucket", ctypes.c_void_p),
+                        ("ninst", ctypes.c_int), ("nitems", ctypes.c_int)]
 
-        args = _Args(plan.table.data_ptr(), plan.items.data_ptr(), plan.nitems)
+        args = _Args(plan.table.data_ptr(), plan.first.data_ptr(), plan.bucket.data_ptr(), plan.instances,
+                     plan.nitems)
         rt.launch(self._function(), (plan.grid, 1, 1), (256, 1, 1), args,
                   stream if stream is not None else default_stream())
 
