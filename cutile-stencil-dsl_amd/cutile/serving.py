@@ -129,6 +129,7 @@ class _Class:
         self.plans = None
         self.fills = None
         self.tick = 0
+        self.stream = None            # the class's own stream in the server
 
     def shape(self, side: int) -> tuple:
         cols = -(-side // self.vz) * self.vz             # rows padded to whole vectors
@@ -291,7 +292,7 @@ class StencilServer:
     job's final fields (device tensors) in ``results`` instead of freeing
     them.  ``inflight``: ticks queued on the device ahead of the host."""
 
-    def __init__(self, hbm_budget: Optional[int] = None, keep_results: bool = False, inflight: int = 2):
+    def __init__(self, hbm_budget: Optional[int] = None, keep_results: bool = False, inflight: int = 3):
         import torch
 
         from cutile.runtime import rt
@@ -306,7 +307,8 @@ class StencilServer:
         self.results: Dict[int, dict] = {}
         self.reserved = 0
         self.peak_reserved = 0
-        self.stream = torch.cuda.current_stream().cuda_stream
+        self.main = torch.cuda.current_stream()
+        self.stream = self.main.cuda_stream
 
     def _class(self, job: Job) -> _Class:
         key = (job.kind, job.dtype)
@@ -389,9 +391,16 @@ class StencilServer:
             host["rebuild"] += time.perf_counter() - tr
             # one tick: every class advances its jobs by one step
             tl = time.perf_counter()
-            for cls in self.classes.values():
-                if cls.jobs:
-                    cls.step(self.stream)
+            # the classes' launches run concurrently on their own streams (so
+            # one class's tail overlaps another's work), joined every tick
+            busy = [c for c in self.classes.values() if c.jobs]
+            for cls in busy:
+                if cls.stream is None:
+                    cls.stream = torch.cuda.Stream()
+                cls.stream.wait_stream(self.main)
+                cls.step(cls.stream.cuda_stream)
+            for cls in busy:
+                self.main.wait_stream(cls.stream)
             ev = rt.Event()
             ev.record(self.stream)
             host["launch"] += time.perf_counter() - tl
