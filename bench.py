@@ -458,7 +458,7 @@ def serving_c5(seconds=10.0, rate=200.0, seed=0):
     return stats
 
 
-def cpu_baseline(n=256, steps=8):
+def cpu_baseline(n=256, steps=32):
     """The NumPy oracle (one core: NumPy ufuncs are single-threaded) on a
     256^3 sample of the headline workload."""
     sys.path.insert(0, ROOT)
