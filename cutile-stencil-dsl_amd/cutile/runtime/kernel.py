@@ -108,17 +108,19 @@ def cu_count() -> int:
     return _cu_cache[dev]
 
 
-def chunk_length(n0: int, tiles: int, default: int, queue_depth: int) -> int:
+def chunk_length(n0: int, tiles: int, default: int, queue_depth: int, per_cu: int = 1) -> int:
     """Planes per workgroup along the streaming axis.
 
-    The grid gets about one workgroup per CU: ``ceil(CUs / tiles)`` chunks of
-    ``ceil(n0 / chunks)`` planes, capped at the compiled chunk ``default``
-    and kept at 4 queue refills or more so the register-queue prologue
-    (``queue_depth - 1`` planes) stays a small fraction of the chunk.  On
-    MI355X at 512^3 one long-streaming workgroup per CU beat both shorter
-    chunks and several resident workgroups per CU for every sweep measured
-    (fewer prologue re-reads, neighbouring tiles advancing in step)."""
-    chunks = max(1, -(-cu_count() // max(1, tiles)))
+    The grid gets about ``per_cu`` workgroups per CU: ``ceil(per_cu * CUs /
+    tiles)`` chunks of ``ceil(n0 / chunks)`` planes, capped at the compiled
+    chunk ``default`` and kept at 4 queue refills or more so the
+    register-queue prologue (``queue_depth - 1`` planes) stays a small
+    fraction of the chunk.  On MI355X one long-streaming workgroup per CU
+    beat shorter chunks and several resident workgroups per CU for the 3-D
+    sweeps at 512^3 (fewer prologue re-reads, neighbouring tiles advancing in
+    step); the one-row tiles of 2-D sweeps ran 3-8 % faster at two per CU
+    (fused Gray-Scott at 4096^2, all four precisions)."""
+    chunks = max(1, -(-(max(1, per_cu) * cu_count()) // max(1, tiles)))
     x = min(max(1, default), -(-n0 // chunks))
     x = max(x, 4 * max(1, queue_depth - 1))
     return max(1, min(x, n0))
@@ -232,7 +234,7 @@ class StencilKernel:
         a.nty = -(-n[1] // ty)
         # a temporal launch warms up T levels: (queue_depth - 1) * T planes per chunk
         depth = self.queue_depth if key != "temporal" else (self.queue_depth - 1) * self.temporal_steps + 1
-        x = xch if xch else chunk_length(n[0], a.ntz * a.nty, self.xch, depth)
+        x = xch if xch else chunk_length(n[0], a.ntz * a.nty, self.xch, depth, 2 if self.ndim == 2 else 1)
         a.nch = -(-n[0] // x)
         a.xch = x
         a.nxcd = xcd_count(current_arch())

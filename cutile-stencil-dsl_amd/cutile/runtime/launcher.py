@@ -386,13 +386,15 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
 def compile_fused(stencil_fns: list, domain=None, hw=None, tile_sizes=None, halo_widths=None,
                   temporal_steps: int = 1, occupancy: Optional[int] = None, align_halo: bool = True,
                   select: Optional[str] = None, storage: Optional[str] = None, update: Optional[str] = None,
-                  dt: float = 1.0, targets: Optional[Sequence[str]] = None) -> CompileResult:
+                  dt: float = 1.0, targets: Optional[Sequence[str]] = None,
+                  kernel_hints: Optional[dict] = None) -> CompileResult:
     """Several stencils over one domain in one kernel (``launcher.py:610-728``):
     inputs merged by parameter name, one load per distinct access, one
     output per member, launcher ``launch_<name>(*inputs, *outputs)``.
     With ``update`` the members are rates of fields ``targets`` and the
     kernel stores the advanced fields (and, for ``"kahan"``, their
-    compensations ``c_<target>``, which are inputs and outputs as well)."""
+    compensations ``c_<target>``, which are inputs and outputs as well).
+    ``kernel_hints`` (as for ``compile``) are merged into every candidate."""
     from cutile.frontend.ir import ApplyOp
     from cutile.lowering.meta import extract_fused_meta
     from cutile.lowering.stencil_to_hip import fit_lds, lower_fused_stencils_to_python
@@ -437,6 +439,8 @@ def compile_fused(stencil_fns: list, domain=None, hw=None, tile_sizes=None, halo
     candidates = [(tuple(tile_sizes), {})] if tile_sizes is not None else tiling.candidates(ndim, num_loads)
     if occupancy is not None:
         candidates = [(candidates[0][0], dict(candidates[0][1], occupancy=occupancy))]
+    if kernel_hints:
+        candidates = [(t, dict(h, **kernel_hints)) for t, h in candidates]
     candidates = [fit_lds(meta, t, h, storage, lds) for t, h in candidates]
 
     def _lower(cfg):

@@ -71,7 +71,7 @@ class TimeStepper:
     by ``apply_boundary``."""
 
     def __init__(self, stencil_fn, boundary, temporal_steps: int = 1, storage: Optional[str] = None,
-                 tile_sizes=None):
+                 tile_sizes=None, kernel_hints: Optional[dict] = None):
         from cutile.runtime.launcher import compile as _compile
 
         self.fn = stencil_fn
@@ -83,7 +83,8 @@ class TimeStepper:
         self.mode = "recompute" if kind == "periodic" else "zero"
         if self.T > 1 and kind == "other":
             self.T = 1  # no exact multi-level semantics for these boundaries
-        self.single = _compile(stencil_fn, temporal_blocking=False, storage=storage, tile_sizes=tile_sizes)
+        self.single = _compile(stencil_fn, temporal_blocking=False, storage=storage, tile_sizes=tile_sizes,
+                               kernel_hints=kernel_hints)
         if self.T > 1:
             self.block = _compile(stencil_fn, temporal_steps=self.T, temporal_mode=self.mode, storage=storage)
             self.halo = self.block.halo_widths
