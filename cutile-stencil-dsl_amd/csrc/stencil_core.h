@@ -99,7 +99,7 @@ template <class F, int... I> ST_HD void for_each_ic(F&& f, iseq<I...>) { (f(ic<I
 
 template <class P> struct Tab {
   static constexpr int NTZ = P::TZ / P::VZ;
-  static constexpr int NTY = P::NT / NTZ;
+  static constexpr int NTY = P::NT / NTZ > 0 ? P::NT / NTZ : 1;
   static constexpr int KY = (P::TY + NTY - 1) / NTY;
   static constexpr bool HASQ = P::QHI >= P::QLO;
   static constexpr int QN = HASQ ? (P::QHI - P::QLO + 1) : 1;
@@ -141,11 +141,12 @@ template <class P> struct Tab {
 };
 
 // Per-cell view handed to the generated expression.  The register queue is
-// a ring: at rotation R, slot (i + R) % QN holds plane x + QLO + i.
-template <class P, int R> struct Cell {
+// a ring: at rotation R, slot (i + R) % QN holds plane x + QLO + i.  KD is
+// the number of cells per thread (the queue's third extent).
+template <class P, int R, int KD = Tab<P>::KY> struct Cell {
   using C = typename P::C;
   using T = Tab<P>;
-  const C (*q)[T::QN][T::KY][P::VZ];
+  const C (*q)[T::QN][KD][P::VZ];
   const C* lds;
   int k, v, ly, lz;
 
@@ -454,6 +455,281 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       if (x + decltype(Rc)::value < xe) step(Rc, x + decltype(Rc)::value);
     }, typename mkseq<QN>::type{});
   }
+}
+
+// ---------------------------------------------------------------------------
+// in-kernel temporal blocking: TT time levels per launch in registers + LDS
+// ---------------------------------------------------------------------------
+//
+// A workgroup owns an output tile of (TY - 2*GY) x (TZ - 2*GZ) cells and a
+// chunk of planes, and evaluates every level on the widened region TY x TZ:
+// the output tile plus GY rows and GZ columns per side (GY >= (TT-1)*RY,
+// GZ >= (TT-1)*RZ), so level l is exact on the output tile widened by
+// (TT-l) radii, which is all level l+1 reads there.  Every level keeps the
+// sweep's structure: a per-thread register queue of QN planes of the
+// thread's own cells (x-neighbours, own-vector z-neighbours) and one LDS
+// plane per (field, dx) buffer for the other in-plane neighbours.  Level 0
+// is the input array: its queue head is loaded from memory and only its LDS
+// halo ring is fetched.  Levels >= 1 never touch memory: a level's queue
+// head is the value the thread has just computed and its LDS planes are
+// copied from its queue.
+//
+// The levels are skewed by D = QHI + 1 planes along the stream axis: at step
+// s, level l (1..TT) evaluates plane s - (l-1)*D from level-(l-1) planes
+// s - (l-1)*D + QLO .. s - (l-1)*D + QHI, all produced in earlier steps.
+// Levels are evaluated from TT down to 1, so each level's queue and LDS
+// planes are read before this step's value of that level is pushed.
+// Level TT is stored.  A chunk of planes [xb, xe) takes steps xb + (TT-1)*QLO
+// .. xe - 1 + (TT-1)*D.
+//
+// Cells are assigned to threads flat: vector cell c = tid + k*NT (k < KC) of
+// the TY x TZ/VZ region in row-major order, so the region's shape needs no
+// relation to the workgroup size (lanes past the region duplicate the last
+// cell and write nothing).
+//
+// Values of intermediate levels outside the interior:
+//   tmode 0: 0 (T chained sweeps through zero-halo scratch arrays);
+//   tmode 1: evaluated like interior cells (T steps with a copy-type halo
+//            refill, when the halo is at least TT radii deep and was filled
+//            before the launch);
+//   a.wrap bit d: axis d is periodic; level-0 reads past the interior wrap
+//            around, so intermediate levels there are exact periodic images
+//            (no halo is read or written along that axis).
+constexpr int pmod(int a, int n) { return ((a % n) + n) % n; }
+
+// index along a periodic axis of interior extent n (any distance)
+ST_HD int wrapm(int i, int n) {
+  const int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+template <class P>
+ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
+  using S = typename P::S;
+  using C = typename P::C;
+  using T = Tab<P>;
+  constexpr int NT = P::NT, TY = P::TY, TZ = P::TZ, VZ = P::VZ, NB = P::NB, TT = P::TT;
+  constexpr int QLO = P::QLO, QHI = P::QHI, QN = T::QN, NVB = T::max_nvb(), LE = T::lds_elems();
+  constexpr int NTZ = TZ / VZ, CELLS = TY * NTZ, KC = (CELLS + NT - 1) / NT;
+  constexpr int D = QHI + 1;
+  constexpr int GY = P::GY, GZ = P::GZ, OTY = TY - 2 * GY, OTZ = TZ - 2 * GZ;
+  constexpr int ES = (int)sizeof(S);
+  static_assert(P::NF == 1 && P::NO == 1 && P::UPD == 0, "temporal blocking: one field, one result");
+  static_assert(T::HASQ && QLO <= 0 && QHI >= 0, "level queues must hold plane offset 0");
+  static_assert(OTY > 0 && OTZ > 0 && GZ % VZ == 0 && TZ % VZ == 0, "temporal tile geometry");
+  static_assert(TT >= 2, "at least two levels");
+
+  __shared__ C lds[TT * LE];
+
+  const int tid = threadIdx.x;
+  const int y0 = tile_y * OTY - GY, z0 = tile_z * OTZ - GZ;   // region origin, interior coordinates
+  const int xb = chunk * a.xch;
+  const int xe = (xb + a.xch < a.n0) ? xb + a.xch : a.n0;
+  const int s1 = (int)a.s1, s2 = (int)a.s2;
+  const i64 plane = a.s1 * a.s2;
+  const int pbytes = (int)(plane * ES);
+  const bool w0 = a.wrap & 1, w1 = a.wrap & 2, w2 = a.wrap & 4;
+  const bool zero_out = a.tmode == 0;
+
+  // my cells, their in-plane load / store offsets and whether levels >= 1
+  // keep their value there (tmode 0 zeroes cells outside the interior)
+  int ly[KC], lz[KC], qoff[KC], soff[KC];
+  bool act[KC], keep[KC];
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const int c = tid + k * NT;
+    act[k] = c < CELLS;
+    const int cc = act[k] ? c : CELLS - 1;
+    ly[k] = cc / NTZ;
+    lz[k] = (cc % NTZ) * VZ;
+    int y = y0 + ly[k], z = z0 + lz[k];
+    const bool iny = y >= 0 && y < a.n1, inz = z >= 0 && z < a.n2;
+    keep[k] = (w1 || iny) && (w2 || inz);
+    const bool mine = act[k] && ly[k] >= GY && ly[k] < GY + OTY && lz[k] >= GZ && lz[k] < GZ + OTZ && iny && inz;
+    soff[k] = mine ? ((a.h1 + y) * s2 + a.h2 + z) * ES : BUF_OOB;
+    if (w1) y = wrapm(y, a.n1);
+    if (w2) z = wrapm(z, a.n2);
+    const bool ok = act[k] && a.h1 + y >= 0 && a.h1 + y < s1 && a.h2 + z >= 0 && a.h2 + z + VZ <= s2;
+    qoff[k] = ok ? ((a.h1 + y) * s2 + a.h2 + z) * ES : BUF_OOB;
+  }
+  // in-plane offsets of the level-0 halo-ring vectors this thread fetches
+  int roff[NB > 0 ? NB : 1][NVB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int j = 0; j < NVB; ++j) {
+      const int i = tid + j * NT;
+      bool ok = (j < T::nvb(b)) && (i < T::gvecs(b));
+      int rr = 0, c = 0;
+      T::gvec_rc(b, ok ? i : 0, rr, c);
+      int gy = y0 - P::BRY[b] + rr, gz = z0 - T::rzv(b) + c * VZ;
+      if (w1) gy = wrapm(gy, a.n1);
+      if (w2) gz = wrapm(gz, a.n2);
+      gy += a.h1;
+      gz += a.h2;
+      ok = ok && gy >= 0 && gy < s1 && gz >= 0 && gz + VZ <= s2;
+      roff[b][j] = ok ? (gy * s2 + gz) * ES : BUF_OOB;
+    }
+  }
+
+  // plane x of the input: its index in the array, and whether it lies there
+  auto in_plane = [&](int x, bool& ok) -> Rsrc {
+    const int px = w0 ? wrapm(x, a.n0) : x;
+    ok = a.h0 + px >= 0 && a.h0 + px < a.s0;
+    return make_rsrc(reinterpret_cast<const S*>(a.in[0]) + (i64)(a.h0 + (ok ? px : xb)) * plane, pbytes);
+  };
+
+  C q[TT][QN][KC][VZ];           // level l's queue (level 0: the input)
+  C pq[KC][VZ];                  // level-0 plane prefetched for the next step
+  C pb[NB > 0 ? NB : 1][NVB][VZ];  // level-0 halo ring prefetched for the next step
+
+  auto load_q0 = [&](int x, C (&dst)[KC][VZ], bool live) {
+    bool ok;
+    const Rsrc r = in_plane(x, ok);
+    ok = ok && live;
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const Vec<S, VZ> w = bload<S, VZ>(r, ok ? qoff[k] : BUF_OOB);
+#pragma unroll
+      for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
+    }
+  };
+  auto load_ring = [&](int x, bool live) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      bool ok;
+      const Rsrc r = in_plane(x + P::BDX[b], ok);
+      ok = ok && live;
+#pragma unroll
+      for (int j = 0; j < NVB; ++j) {
+        const Vec<S, VZ> w = bload<S, VZ>(r, ok ? roff[b][j] : BUF_OOB);
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) pb[b][j][v] = Cvt<S, C>::ld(w.v[v]);
+      }
+    }
+  };
+
+  const int s_first = xb + (TT - 1) * QLO;
+  const int s_last = xe - 1 + (TT - 1) * D;
+
+  // ---- prologue: level-0 planes s_first+QLO .. s_first+QHI-1 in slots
+  //      0 .. QN-2, plane s_first+QHI and the ring of plane s_first
+  //      prefetched; levels >= 1 and the LDS start at 0 (the halo rings of
+  //      levels >= 1 are never written)
+#pragma unroll
+  for (int i = 0; i + 1 < QN; ++i) load_q0(s_first + QLO + i, q[0][i], true);
+  load_q0(s_first + QHI, pq, true);
+  load_ring(s_first, true);
+#pragma unroll
+  for (int l = 1; l < TT; ++l)
+#pragma unroll
+    for (int i = 0; i < QN; ++i)
+#pragma unroll
+      for (int k = 0; k < KC; ++k)
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) q[l][i][k][v] = C(0);
+  for (int i = tid; i < TT * LE; i += NT) lds[i] = C(0);
+
+  S* out = reinterpret_cast<S*>(a.out[0]);
+
+  // one step; R = (s - s_first) % QN.  Plane p of level l sits in queue
+  // slot (p - QLO - s_first) mod QN.
+  auto step = [&](auto Rc, int s) {
+    constexpr int R = decltype(Rc)::value;
+    // level-0 head: plane s + QHI
+#pragma unroll
+    for (int k = 0; k < KC; ++k)
+#pragma unroll
+      for (int v = 0; v < VZ; ++v) q[0][pmod(R - 1, QN)][k][v] = pq[k][v];
+    __syncthreads();
+    // LDS planes: level l's buffer b holds plane s - l*D + BDX[b] of level l
+    for_each_ic([&](auto Lc) {
+      constexpr int L = decltype(Lc)::value;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        C* base = lds + L * LE + T::off(b);
+        const int sl = pmod(R - L * D + P::BDX[b] - QLO, QN);
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          if (act[k]) {
+            C* d = base + (P::BRY[b] + ly[k]) * T::cols(b) + T::rzv(b) + lz[k];
+#pragma unroll
+            for (int v = 0; v < VZ; ++v) d[v] = q[L][sl][k][v];
+          }
+        }
+        if constexpr (L == 0) {
+#pragma unroll
+          for (int j = 0; j < NVB; ++j) {
+            const int i = tid + j * NT;
+            if (j < T::nvb(b) && i < T::gvecs(b)) {
+              int rr, c;
+              T::gvec_rc(b, i, rr, c);
+              C* d = base + rr * T::cols(b) + c * VZ;
+#pragma unroll
+              for (int v = 0; v < VZ; ++v) d[v] = pb[b][j][v];
+            }
+          }
+        }
+      }
+    }, typename mkseq<TT>::type{});
+    __syncthreads();
+    // prefetch the next step's level-0 plane and ring while this step computes
+    load_q0(s + 1 + QHI, pq, s + 1 <= s_last);
+    load_ring(s + 1, s + 1 <= s_last);
+
+    // levels TT .. 1
+    for_each_ic([&](auto Mc) {
+      constexpr int L = TT - decltype(Mc)::value;     // level evaluated
+      constexpr int RL = pmod(R - (L - 1) * D, QN);   // rotation of level L-1's queue
+      const int p = s - (L - 1) * D;                  // plane evaluated
+      C res[KC][VZ];
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) {
+          Cell<P, RL, KC> c{&q[L - 1], lds + (L - 1) * LE, k, v, ly[k], lz[k] + v};
+          C o[1];
+          P::eval(c, o);
+          res[k][v] = o[0];
+        }
+      }
+      if constexpr (L == TT) {
+        const bool st = p >= xb;
+        const Rsrc ro = make_rsrc(out + (i64)(a.h0 + (st ? p : xb)) * plane, pbytes);
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          Vec<S, VZ> w;
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) w.v[v] = Cvt<S, C>::st(res[k][v]);
+          bstore<S, VZ, P::NTS ? BUF_NT : 0>(w, ro, st ? soff[k] : BUF_OOB);
+        }
+      } else {
+        constexpr int sl = pmod(R - (L - 1) * D - QLO, QN);
+        const bool pin = w0 || (p >= 0 && p < a.n0);
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          const bool zero = zero_out && !(pin && keep[k]);
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) q[L][sl][k][v] = zero ? C(0) : res[k][v];
+        }
+      }
+    }, typename mkseq<TT>::type{});
+  };
+
+  for (int s = s_first; s <= s_last; s += QN) {
+    for_each_ic([&](auto Rc) {
+      if (s + decltype(Rc)::value <= s_last) step(Rc, s + decltype(Rc)::value);
+    }, typename mkseq<QN>::type{});
+  }
+}
+
+template <class P>
+ST_HD void tblock_entry(const SweepArgs& a) {
+  const int n = a.ntz * a.nty * a.nch;
+  const int t = xcd_swizzle((int)blockIdx.x, n, a.nxcd);
+  const int tile_z = t % a.ntz;
+  const int rest = t / a.ntz;
+  tblock_tile<P>(a, tile_z, rest % a.nty, rest / a.nty);
 }
 
 // ---------------------------------------------------------------------------

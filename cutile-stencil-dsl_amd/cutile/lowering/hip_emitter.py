@@ -99,23 +99,141 @@ def temporal_tile(ndim: int, T: int, r3: tuple, compute: str, budget: int = 160 
     raise ValueError(f"temporal depth {T} does not fit the LDS budget for radius {r3}")
 
 
+@dataclass(frozen=True)
+class TBlockConfig:
+    """Geometry of one register/LDS temporal-blocking variant.
+
+    The workgroup evaluates ``T`` levels on a ``ty`` x ``tz`` region: its
+    output tile widened by ``gy`` rows and ``gz`` columns on each side.
+    ``nt`` threads own the region's ``ty * tz / vz`` vector cells flat,
+    ``kc`` each (the last ones may be idle)."""
+
+    ty: int
+    tz: int
+    vz: int
+    nt: int
+    gy: int
+    gz: int
+    T: int
+
+    @property
+    def oty(self) -> int:
+        return self.ty - 2 * self.gy
+
+    @property
+    def otz(self) -> int:
+        return self.tz - 2 * self.gz
+
+    @property
+    def kc(self) -> int:
+        return -(-(self.ty * (self.tz // self.vz)) // self.nt)
+
+    @property
+    def tag(self) -> str:
+        return f"tb{self.T}r{self.ty}x{self.tz}v{self.vz}n{self.nt}"
+
+    def kernel_config(self) -> KernelConfig:
+        return KernelConfig(self.ty, self.tz, self.vz, self.nt)
+
+
+def tblock_queue_range(meta: StencilMeta) -> tuple:
+    """Plane offsets every level's register queue holds: all accessed dx and
+    0 (levels >= 1 fill their LDS planes from the queue)."""
+    plan = plan_accesses(meta)
+    dxs = [0] + [b[1] for b in plan["bufs"]]
+    if plan["qhi"] >= plan["qlo"]:
+        dxs += [plan["qlo"], plan["qhi"]]
+    return min(dxs), max(dxs)
+
+
+def _vgpr_limit(nt: int) -> int:
+    """Architectural VGPRs a wave may use so that a workgroup of ``nt``
+    threads fits one CU (4 SIMDs, 512 VGPRs per lane each)."""
+    waves_per_simd = -(-nt // 256)
+    return min(256, 512 // waves_per_simd)
+
+
+def tblock_estimate(meta: StencilMeta, cfg: TBlockConfig, storage: str, compute: str) -> dict:
+    """LDS bytes and a VGPR estimate (queues, prefetch, ring, offsets) of a
+    temporal-blocking variant."""
+    plan = plan_accesses(meta)
+    qlo, qhi = tblock_queue_range(meta)
+    qn = qhi - qlo + 1
+    cw = ELEM_BYTES[compute] // 4
+    lds = cfg.T * lds_bytes(meta, cfg.kernel_config(), compute)
+    nvb = 1
+    for f, dx, ry, rz in plan["bufs"]:
+        rzv = -(-rz // cfg.vz) * cfg.vz
+        w = (cfg.tz + 2 * rzv) // cfg.vz
+        ring = 2 * ry * w + cfg.ty * 2 * (rzv // cfg.vz)
+        nvb = max(nvb, -(-ring // cfg.nt))
+    nb = max(1, len(plan["bufs"]))
+    # queues of T levels + prefetched plane and ring + per-cell offsets and
+    # flags + the expression's working set (its loaded operands)
+    work = cw * (min(len(access_table(meta)), 40) + 8) + 24
+    regs = cw * cfg.vz * cfg.kc * (cfg.T * qn + 1) + cw * cfg.vz * nb * nvb + 16 * cfg.kc + nb * nvb + work
+    return {"lds": lds, "vgpr": regs, "qn": qn}
+
+
+def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int,
+                  budget: int = 160 * 1024, max_otz: int = 256) -> "TBlockConfig | None":
+    """Cheapest register/LDS temporal-blocking geometry for ``T`` levels at
+    vector width ``vz``: fewest lane-cells evaluated per output cell (ghost
+    ring and idle lanes included) whose queues fit the VGPR budget of its
+    workgroup size and whose level planes fit the LDS.  None when nothing
+    fits (deep ghosts of wide stencils); the caller falls back to the
+    LDS-ring kernel."""
+    if meta.num_inputs != 1 or meta.num_exprs != 1 or meta.update or T < 2:
+        return None
+    rx, ry, rz = radius3(meta)
+    nd = meta.ndim
+    gy = (T - 1) * ry
+    gz = -(-((T - 1) * rz) // vz) * vz
+    best = None
+    otzs = [t for t in (16, 32, 64, 128, 256, 512, 1024) if t <= max_otz and t % vz == 0]
+    otys = range(1, 65) if nd == 3 else (1,)
+    for otz in otzs:
+        tz = otz + 2 * gz
+        for oty in otys:
+            ty = oty + 2 * gy
+            cells = ty * (tz // vz)
+            for nt in range(64, 1025, 64):
+                kc = -(-cells // nt)
+                if kc > 4:
+                    continue
+                if kc > 1 and (kc - 1) * nt >= cells:
+                    continue
+                cfg = TBlockConfig(ty, tz, vz, nt, gy, gz, T)
+                est = tblock_estimate(meta, cfg, storage, compute)
+                if est["lds"] > budget or est["vgpr"] > 0.85 * _vgpr_limit(nt):
+                    continue
+                cost = kc * nt * vz / (oty * otz)
+                key = (round(cost, 3), -(oty * otz), nt)
+                if best is None or key < best[0]:
+                    best = (key, cfg)
+    return best[1] if best else None
+
+
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,
                 nontemporal: bool = True, prefetch: int = 1, partial: bool = False,
-                ntload: bool = False) -> str:
+                ntload: bool = False, tblock: "TBlockConfig | None" = None) -> str:
     """Traits struct of one kernel variant.  ``partial``: rows may end in a
     partial vector (batched instances whose interior is not a multiple of
-    the vector width)."""
+    the vector width).  ``tblock``: traits of a temporal-blocking variant
+    (levels, ghost widths; the queue holds every accessed plane offset)."""
     plan = plan_accesses(meta)
     bufs = plan["bufs"]
     r3 = radius3(meta)
+    qlo, qhi = (plan["qlo"], plan["qhi"]) if tblock is None else tblock_queue_range(meta)
+    qf = plan["qf"] if tblock is None else [1] * meta.num_inputs
     lines = [
         f"struct P_{tag} {{",
         f"  using S = {STORAGE_CTYPES[storage]};",
         f"  using C = {COMPUTE_CTYPES[compute]};",
         f"  static constexpr int NF = {meta.num_inputs}, NO = {meta.num_exprs};",
         f"  static constexpr int NT = {cfg.nt}, TY = {cfg.ty}, TZ = {cfg.tz}, VZ = {cfg.vz};",
-        f"  static constexpr int QLO = {plan['qlo']}, QHI = {plan['qhi']};",
-        f"  static constexpr int QF[{max(1, meta.num_inputs)}] = {_arr(plan['qf'])};",
+        f"  static constexpr int QLO = {qlo}, QHI = {qhi};",
+        f"  static constexpr int QF[{max(1, meta.num_inputs)}] = {_arr(qf)};",
         f"  static constexpr int NB = {len(bufs)};",
         f"  static constexpr int BF[{max(1, len(bufs))}] = {_arr([b[0] for b in bufs])};",
         f"  static constexpr int BDX[{max(1, len(bufs))}] = {_arr([b[1] for b in bufs])};",
@@ -125,6 +243,8 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int NTS = {1 if nontemporal else 0}, PD = {int(prefetch)}, "
         f"PARTIAL = {1 if partial else 0}, NTL = {1 if ntload else 0};",
     ]
+    if tblock is not None:
+        lines.append(f"  static constexpr int TT = {tblock.T}, GY = {tblock.gy}, GZ = {tblock.gz};")
     upd = meta.update
     ne = meta.num_exprs
     if upd is None:
@@ -155,10 +275,16 @@ def temporal_symbol(meta: StencilMeta, tag: str, T: int) -> str:
     return f"{meta.name}_temporal_tb{T}_{tag}"
 
 
+def tblock_symbol(meta: StencilMeta, cfg: TBlockConfig) -> str:
+    return f"{meta.name}_tblock_{cfg.tag}"
+
+
 def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage: str,
                     compute: str, temporal=None, nontemporal: bool = True, prefetch: int = 1,
-                    ntload: bool = False) -> str:
-    """Full HIP translation unit with one sweep kernel per variant."""
+                    ntload: bool = False, tblocks: Sequence[TBlockConfig] = ()) -> str:
+    """Full HIP translation unit with one sweep kernel per variant (plus the
+    temporal-blocking kernels: ``temporal`` = (config, T) of the LDS-ring
+    kernel, ``tblocks`` the register/LDS ones)."""
     out = [
         f"// {meta.name}: generated by the HIP emitter for gfx950",
         f"// expression(s): " + " | ".join(meta.py_expressions),
@@ -180,6 +306,14 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
         out.append(
             f'extern "C" __global__ void __launch_bounds__({cfg.nt}) '
             f"{temporal_symbol(meta, cfg.tag, T)}(st::SweepArgs a) {{ st::temporal_entry<P_{tag}, {T}>(a); }}"
+        )
+        out.append("")
+    for tb in tblocks:
+        tag = tb.tag
+        out.append(emit_traits(meta, tb.kernel_config(), storage, compute, tag, nontemporal, tblock=tb))
+        out.append(
+            f'extern "C" __global__ void __launch_bounds__({tb.nt}) '
+            f"{tblock_symbol(meta, tb)}(st::SweepArgs a) {{ st::tblock_entry<P_{tag}>(a); }}"
         )
         out.append("")
     return "\n".join(out)

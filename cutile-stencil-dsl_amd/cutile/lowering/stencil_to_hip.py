@@ -27,7 +27,8 @@ from typing import Optional, Sequence
 from cutile.frontend.ir import find_func
 from cutile.lowering.emitter import CodeEmitter
 from cutile.lowering.hip_emitter import (ELEM_BYTES, KernelConfig, emit_hip_source, kernel_symbol, lds_bytes,
-                                        plan_accesses, radius3, temporal_symbol, temporal_tile)
+                                        plan_accesses, radius3, tblock_config, tblock_queue_range,
+                                        tblock_symbol, temporal_symbol, temporal_tile)
 from cutile.lowering.meta import StencilMeta, apply_update, extract_fused_meta, extract_meta
 
 
@@ -116,15 +117,30 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
     vec, scal, xch = make_configs(meta.ndim, tile_sizes, storage_t, (kernel_hints or {}).get("threads"),
                                   (kernel_hints or {}).get("vector"))
     temporal = None
+    tblocks = []
+    tblock_depth = 0
     if temporal_steps > 1:
         if meta.num_inputs != 1 or meta.num_outputs != 1:
             raise NotImplementedError("in-kernel temporal blocking needs one input and one output")
-        ty, tz = temporal_tile(meta.ndim, temporal_steps, radius3(meta), compute_t)
-        temporal = (KernelConfig(ty=ty, tz=tz, vz=1, nt=256), int(temporal_steps))
+        # register/LDS kernel at the vector width and at scalar width (arrays
+        # whose rows are not vector-aligned); the LDS-ring kernel when the
+        # ghost ring of T levels does not fit the registers or the LDS
+        for vz in dict.fromkeys((min(vec.vz, 4), 1)):
+            tb = tblock_config(meta, int(temporal_steps), storage_t, compute_t, vz)
+            if tb is None:
+                tblocks = []
+                break
+            tblocks.append(tb)
+        if tblocks:
+            qlo, qhi = tblock_queue_range(meta)
+            tblock_depth = int(temporal_steps) * (qhi - qlo + 1)
+        else:
+            ty, tz = temporal_tile(meta.ndim, temporal_steps, radius3(meta), compute_t)
+            temporal = (KernelConfig(ty=ty, tz=tz, vz=1, nt=256), int(temporal_steps))
     src = emit_hip_source(meta, [vec, scal], storage_t, compute_t, temporal,
                           nontemporal=bool((kernel_hints or {}).get("nontemporal", 1)),
                           prefetch=int((kernel_hints or {}).get("prefetch", 1)),
-                          ntload=bool((kernel_hints or {}).get("ntload", 0)))
+                          ntload=bool((kernel_hints or {}).get("ntload", 0)), tblocks=tblocks)
     attr = _attrs(kernel_hints)
     if attr:
         src = src.replace('extern "C" __global__ void ', f'extern "C" __global__ void {attr}')
@@ -135,6 +151,9 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
     if temporal is not None:
         tcfg, T = temporal
         variants["temporal"] = (temporal_symbol(meta, tcfg.tag, T), tcfg.ty, tcfg.tz, 1, tcfg.nt)
+    for tb, key in zip(tblocks, ("tblock", "tblock_s")):
+        # (symbol, output tile rows, output tile columns, vector, threads)
+        variants[key] = (tblock_symbol(meta, tb), tb.oty, tb.otz, tb.vz, tb.nt)
     plan = plan_accesses(meta)
     qdepth = max(1, plan["qhi"] - plan["qlo"] + 1)
     nd = meta.ndim
@@ -175,7 +194,8 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
     e.line(f"_VARIANTS = {json.dumps(variants)}")
     e.line(f"_KERNEL = StencilKernel({meta.name!r}, {nd}, {meta.num_inputs}, {meta.num_outputs}, "
            f"_HIP_SOURCE, {{k: tuple(v) for k, v in _VARIANTS.items()}}, STORAGE, {int(xch)}, "
-           f"queue_depth={qdepth}, temporal_steps={int(temporal[1]) if temporal is not None else 1})")
+           f"queue_depth={qdepth}, temporal_steps={int(temporal_steps) if temporal_steps > 1 else 1}, "
+           f"tblock_depth={tblock_depth})")
     e.line(f"_HALO = ({', '.join(hvars)}{tr})")
 
     if boundary_spec is not None and meta.num_outputs == 1 and not fused:

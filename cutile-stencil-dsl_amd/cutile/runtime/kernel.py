@@ -126,6 +126,16 @@ def chunk_length(n0: int, tiles: int, default: int, queue_depth: int, per_cu: in
     return max(1, min(x, n0))
 
 
+def tblock_chunk_length(n0: int, tiles: int, default: int, depth: int) -> int:
+    """Planes per workgroup of a temporal-blocking launch: about two
+    workgroups per CU, at least ``depth`` planes (the warm-up of the skewed
+    levels costs ``depth`` steps per chunk, so shorter chunks would spend
+    most of their time warming up)."""
+    chunks = max(1, -(-(2 * cu_count()) // max(1, tiles)))
+    x = min(max(1, default), -(-n0 // chunks))
+    return max(1, min(max(x, depth), n0))
+
+
 def xcd_count(arch: str) -> int:
     # CDNA3/CDNA4 parts are built from 8 XCDs, each with its own L2
     return 8 if arch.startswith(("gfx942", "gfx950")) else 1
@@ -135,8 +145,11 @@ class StencilKernel:
     """A compiled-on-demand family of sweep variants for one stencil."""
 
     def __init__(self, name: str, ndim: int, num_inputs: int, num_outputs: int, source: str,
-                 variants: dict, storage: str, xch: int, queue_depth: int = 3, temporal_steps: int = 1):
-        # variants: {"vec": (symbol, ty, tz, vz, nt), "scalar": (...)}
+                 variants: dict, storage: str, xch: int, queue_depth: int = 3, temporal_steps: int = 1,
+                 tblock_depth: int = 0):
+        # variants: {"vec": (symbol, ty, tz, vz, nt), "scalar": (...)}; temporal blocking:
+        # "tblock" / "tblock_s" (register/LDS kernel, vector / scalar; ty, tz = output tile)
+        # or "temporal" (LDS-ring kernel)
         self.name = name
         self.ndim = ndim
         self.num_inputs = num_inputs
@@ -147,6 +160,7 @@ class StencilKernel:
         self.xch = xch
         self.queue_depth = queue_depth
         self.temporal_steps = temporal_steps
+        self.tblock_depth = tblock_depth
         self._modules: dict = {}
         self._fns: dict = {}
         self._lock = threading.Lock()
@@ -165,6 +179,11 @@ class StencilKernel:
                                             name=f"{self.name}.hip")
                     self._modules[dev] = mod
                 fn = rt.get_function(mod, self.variants[key][0])
+                if key.startswith("tblock"):
+                    res = rt.function_resources(fn)
+                    if res.local_bytes > 0:
+                        raise RuntimeError(f"{self.variants[key][0]}: {res.local_bytes} B of scratch per thread "
+                                           f"(register spills); not launched")
                 self._fns[k] = fn
         return fn
 
@@ -195,16 +214,25 @@ class StencilKernel:
                 if o.data_ptr() == i.data_ptr():
                     raise ValueError(f"launch_{self.name}: output aliases an input (out-of-place only)")
 
-    def pick(self, inputs, outputs, halo) -> str:
+    def vector_ok(self, inputs, outputs, halo, vz: int) -> bool:
+        """Rows of every array start and end on a ``vz``-element boundary."""
         s = map3_shape(self.ndim, inputs[0].shape)
         h = map3_shape(self.ndim, halo) if self.ndim != 1 else (0, 0, halo[0])
         if self.ndim == 2:
             h = (halo[0], 0, halo[1])
-        _, _, _, vz, _ = self.variants["vec"]
         ok = (h[2] % vz == 0) and (s[2] % vz == 0)
         esz = inputs[0].element_size()
-        ok = ok and all((t.data_ptr() % (vz * esz)) == 0 for t in list(inputs) + list(outputs))
-        return "vec" if ok else "scalar"
+        return ok and all((t.data_ptr() % (vz * esz)) == 0 for t in list(inputs) + list(outputs))
+
+    def pick(self, inputs, outputs, halo) -> str:
+        return "vec" if self.vector_ok(inputs, outputs, halo, self.variants["vec"][3]) else "scalar"
+
+    def temporal_key(self, inputs, outputs, halo) -> str:
+        if "tblock" in self.variants:
+            if self.vector_ok(inputs, outputs, halo, self.variants["tblock"][3]):
+                return "tblock"
+            return "tblock_s"
+        return "temporal"
 
     yslab = 0
 
@@ -232,9 +260,13 @@ class StencilKernel:
         a.n0, a.n1, a.n2 = n
         a.ntz = -(-n[2] // tz)
         a.nty = -(-n[1] // ty)
-        # a temporal launch warms up T levels: (queue_depth - 1) * T planes per chunk
-        depth = self.queue_depth if key != "temporal" else (self.queue_depth - 1) * self.temporal_steps + 1
-        x = xch if xch else chunk_length(n[0], a.ntz * a.nty, self.xch, depth, 2 if self.ndim == 2 else 1)
+        if key.startswith("tblock"):
+            # T levels skewed by a queue length each: depth planes of warm-up per chunk
+            x = xch if xch else tblock_chunk_length(n[0], a.ntz * a.nty, self.xch, self.tblock_depth)
+        else:
+            # the LDS-ring temporal kernel warms up (queue_depth - 1) * T planes per chunk
+            depth = self.queue_depth if key != "temporal" else (self.queue_depth - 1) * self.temporal_steps + 1
+            x = xch if xch else chunk_length(n[0], a.ntz * a.nty, self.xch, depth, 2 if self.ndim == 2 else 1)
         a.nch = -(-n[0] // x)
         a.xch = x
         a.nxcd = xcd_count(current_arch())
@@ -242,27 +274,37 @@ class StencilKernel:
         return a, a.ntz * a.nty * a.nch
 
     def launch_temporal(self, inputs: Sequence, outputs: Sequence, halo: Sequence[int], stream=None,
-                        mode: int = 0, xch: Optional[int] = None, validate: bool = True) -> None:
+                        mode: int = 0, xch: Optional[int] = None, validate: bool = True,
+                        periodic_axes: Sequence[int] = ()) -> None:
         """Advance ``T`` steps in one launch (in-kernel temporal blocking).
         ``mode`` 0: intermediate levels are zero outside the interior (the
         semantics of T chained sweeps through zero-halo scratch arrays);
         1: intermediate levels are recomputed in a halo at least T*r deep
-        (T steps with a periodic refill when the halo was filled first)."""
+        (T steps with a periodic refill when the halo was filled first).
+        ``periodic_axes`` (register/LDS kernel only): axes along which reads
+        past the interior wrap around, so T periodic steps need no halo
+        at all; the output's halo is not written."""
         if validate:
             self.check_arrays(inputs, outputs)
-        args, nblocks = self.args(inputs, outputs, halo, "temporal", xch)
+        key = self.temporal_key(inputs, outputs, halo)
+        args, nblocks = self.args(inputs, outputs, halo, key, xch)
         if args is None:
             return
         args.tmode = int(mode)
-        fn = self._function("temporal")
+        if periodic_axes:
+            if not key.startswith("tblock"):
+                raise ValueError(f"launch_{self.name}: periodic reads need the register/LDS temporal kernel")
+            args.wrap = self.wrap_mask(periodic_axes, args, need_halo=False)
+        fn = self._function(key)
         if stream is None:
             stream = default_stream()
-        rt.launch(fn, (nblocks, 1, 1), (self.variants["temporal"][4], 1, 1), args, stream)
+        rt.launch(fn, (nblocks, 1, 1), (self.variants[key][4], 1, 1), args, stream)
 
-    def wrap_mask(self, periodic_axes: Sequence[int], args) -> int:
+    def wrap_mask(self, periodic_axes: Sequence[int], args, need_halo: bool = True) -> int:
         """Bits of the 3-D map for the periodic array axes (the sweep wraps
-        reads past the interior instead of reading the halo); every such
-        axis needs an interior at least as wide as its halo."""
+        reads past the interior instead of reading the halo); for the sweep
+        every such axis needs an interior at least as wide as its halo (the
+        temporal kernel wraps any distance)."""
         if self.ndim == 1:
             bits = {0: 4}
         elif self.ndim == 2:
@@ -273,7 +315,7 @@ class StencilKernel:
         for ax in periodic_axes:
             b = bits[int(ax)]
             n, h = {1: (args.n0, args.h0), 2: (args.n1, args.h1), 4: (args.n2, args.h2)}[b]
-            if n < h:
+            if need_halo and n < h:
                 raise ValueError(f"periodic reads in the sweep need interior >= halo on axis {ax} "
                                  f"(interior {n}, halo {h})")
             mask |= b

@@ -114,13 +114,23 @@ class TimeStepper:
             self.fill(b, stream)
         return b, a
 
-    def _blk(self, a, b, stream):
-        # the halo of `a` is current: every step or block ends with a fill
-        self._block.launch_temporal([a], [b], self.halo, stream, mode=1 if self.mode == "recompute" else 0,
-                                    validate=False)
-        if self.policy.mode != "static":   # a Dirichlet halo is never written: it stays valid
+    def _blk(self, a, b, stream, wrap=()):
+        # the halo of `a` is current (every step or block ends with a fill),
+        # or `wrap` names the periodic axes the block reads around
+        self._block.launch_temporal([a], [b], self.halo, stream,
+                                    mode=1 if (self.mode == "recompute" and not wrap) else 0,
+                                    validate=False, periodic_axes=wrap)
+        if not wrap and self.policy.mode != "static":   # a Dirichlet halo is never written: it stays valid
             self.fill(b, stream)
         return b, a
+
+    def _block_wrap(self) -> tuple:
+        """Periodic axes a temporal block reads around (no halo fills between
+        blocks): all of them for a periodic problem on the register/LDS
+        kernel, which wraps any distance."""
+        if self.policy.mode == "wrap" and "tblock" in self._block.variants:
+            return tuple(range(self.ndim))
+        return ()
 
     def enqueue(self, a, b, steps: int, stream=None):
         """Enqueue ``steps`` steps from ``a`` (halo already valid) using ``b``
@@ -131,15 +141,22 @@ class TimeStepper:
             stream = default_stream()
         if self.policy.mode == "static":
             self.fill(b, stream)          # constant halo: valid in both buffers from here on
+        stale = False                     # the halo of `a` is out of date (wrapped launches)
         if self._block is not None:
-            for _ in range(steps // self.T):
-                a, b = self._blk(a, b, stream)
-            steps %= self.T
+            bw = self._block_wrap()
+            nblk = steps // self.T
+            for _ in range(nblk):
+                a, b = self._blk(a, b, stream, bw)
+            stale = bool(bw) and nblk > 0
+            steps -= nblk * self.T
         axes = self.policy.axes_for(a)
+        if stale and steps and not axes:
+            self.fill(a, stream)
+            stale = False
         for _ in range(steps):
             a, b = self._one(a, b, stream, axes)
-        if axes and steps:
-            self.fill(a, stream)          # the wrapped sweeps left the halo stale
+        if stale or (axes and steps):
+            self.fill(a, stream)          # wrapped launches left the halo stale
         return a, b
 
     def march(self, u, steps: int, scratch=None, stream=None, graph: bool = False, graph_steps: int = 0):
@@ -171,7 +188,7 @@ class TimeStepper:
         # warm the JIT outside the capture
         self._single._function(self._single.pick([u], [b], self.halo))
         if self._block is not None:
-            self._block._function("temporal")
+            self._block._function(self._block.temporal_key([u], [b], self.halo))
         chunk = graph_steps or steps
         per = chunk if self._block is None else (chunk // self.T) * self.T or chunk
         reps, rest = divmod(steps, per)
@@ -187,6 +204,10 @@ class TimeStepper:
                 if a2 is not a:
                     a.copy_(a2)
             self._graph = g
+            # the graph reads and writes these buffers on every replay: they
+            # must outlive it (a freed scratch would be handed to the next
+            # allocation while replays still write into it)
+            self._graph_buffers = (a, b, a2, b2)
         if rest:
             a, _ = self.enqueue(a, b if b is not a else torch.empty_like(a), rest, stream)
         return a

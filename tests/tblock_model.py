@@ -1,0 +1,259 @@
+"""Host-side model of the register/LDS temporal-blocking kernel.
+
+Mirrors ``st::tblock_tile`` in ``csrc/stencil_core.h`` step by step at cell
+granularity: the per-level register queues and their slot rotation, the LDS
+planes of every level (interior copied from the queue, level-0 halo ring
+fetched from memory), the skew of the levels along the stream axis, the
+masking of intermediate levels, and the stores of the last level.  Every
+global access is checked against the array (plane inside the array, vector
+inside the plane) and every LDS read against its buffer, and the values it
+produces are returned so the tests can compare them with the oracle before
+the kernel runs on a device.
+"""
+
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+import oracle
+
+
+def _pmod(a, n):
+    return ((a % n) + n) % n
+
+
+def _wrapm(i, n):
+    return np.mod(i, n)
+
+
+def map3(ndim, seq):
+    seq = tuple(seq)
+    if ndim == 1:
+        return (1, 1, seq[0])
+    if ndim == 2:
+        return (seq[0], 1, seq[1])
+    return seq
+
+
+def halo3(ndim, halo):
+    halo = tuple(halo)
+    if ndim == 1:
+        return (0, 0, halo[0])
+    if ndim == 2:
+        return (halo[0], 0, halo[1])
+    return halo
+
+
+def _offset3(ndim, offs):
+    offs = tuple(offs)
+    if ndim == 1:
+        return (0, 0, offs[0])
+    if ndim == 2:
+        return (offs[0], 0, offs[1])
+    return offs
+
+
+class _Proxy:
+    """The stencil's field argument: u[i+a, j+b] -> the level's values at
+    that offset for every cell of the region."""
+
+    def __init__(self, ndim, get):
+        self.ndim, self.get = ndim, get
+
+    def __getitem__(self, key):
+        if not isinstance(key, tuple):
+            key = (key,)
+        return self.get(*_offset3(self.ndim, key))
+
+
+def gvec_rc(b, i):
+    """Python copy of ``Tab::gvec_rc`` for a buffer whose interior comes from
+    the queue (every temporal buffer does)."""
+    w = b["w"]
+    band = b["ry"] * w
+    if i < band:
+        return i // w, i % w
+    i -= band
+    if i < band:
+        return b["ry"] + b["TY"] + i // w, i % w
+    i -= band
+    sv = b["sv"]
+    r = b["ry"] + i // (2 * sv)
+    k = i % (2 * sv)
+    return r, (k if k < sv else w - 2 * sv + k)
+
+
+def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, edges=False):
+    """Run one launch of the temporal kernel on ``u`` (the padded input).
+
+    ``cfg``: the variant's TBlockConfig; ``plan``: ``plan_accesses`` of the
+    stencil; ``qrange``: ``tblock_queue_range``.  ``edges``: only the
+    workgroups of the first two and last two tiles / chunks along each axis
+    (bounds of a large launch; the output is then partial).  Returns
+    (output array, list of error strings)."""
+    TT, TY, TZ, VZ, NT = cfg.T, cfg.ty, cfg.tz, cfg.vz, cfg.nt
+    GY, GZ, OTY, OTZ = cfg.gy, cfg.gz, cfg.oty, cfg.otz
+    QLO, QHI = qrange
+    QN, D = QHI - QLO + 1, QHI + 1
+    errs = []
+    u3 = np.asarray(u).reshape(map3(ndim, u.shape))
+    s0, s1, s2 = u3.shape
+    h0, h1, h2 = halo3(ndim, halo)
+    n0, n1, n2 = s0 - 2 * h0, s1 - 2 * h1, s2 - 2 * h2
+    w0, w1, w2 = bool(wrap & 1), bool(wrap & 2), bool(wrap & 4)
+    res = np.array(u3, copy=True) if out is None else np.asarray(out).reshape(u3.shape)
+    written = np.zeros(u3.shape, dtype=np.int32)
+    f = oracle.array_aware(fn)
+    bufs = []
+    for (fi, dx, ry, rz) in plan["bufs"]:
+        rzv = -(-rz // VZ) * VZ
+        cols = TZ + 2 * rzv
+        w = cols // VZ
+        sv = rzv // VZ
+        bufs.append(dict(dx=dx, ry=ry, rz=rz, rzv=rzv, rows=TY + 2 * ry, cols=cols, w=w, sv=sv, TY=TY,
+                         gvecs=2 * ry * w + TY * 2 * sv))
+    bidx = {b["dx"]: k for k, b in enumerate(bufs)}
+    cells = TY * (TZ // VZ)
+    if -(-cells // NT) != cfg.kc:
+        errs.append("cells per thread")
+    ntz, nty, nch = -(-n2 // OTZ), -(-n1 // OTY), -(-n0 // xch)
+    ly = np.arange(TY)[:, None] * np.ones((1, TZ), dtype=int)
+    lz = np.ones((TY, 1), dtype=int) * np.arange(TZ)[None, :]
+    lzv = (lz // VZ) * VZ                       # first cell of each cell's vector
+
+    def plane(x):
+        px = int(_wrapm(x, n0)) if w0 else x
+        return px, (0 <= h0 + px < s0)
+
+    def pick(n):
+        return sorted({0, 1, n - 2, n - 1} & set(range(n))) if edges else range(n)
+
+    for tile_z, tile_y, chunk in itertools.product(pick(ntz), pick(nty), pick(nch)):
+        y0, z0 = tile_y * OTY - GY, tile_z * OTZ - GZ
+        xb = chunk * xch
+        xe = min(xb + xch, n0)
+        y = y0 + ly
+        z = z0 + lz
+        zv = z0 + lzv
+        iny, inz = (y >= 0) & (y < n1), (z >= 0) & (z < n2)
+        keep = (w1 | iny) & (w2 | inz)
+        yq = _wrapm(y, n1) if w1 else y
+        zq = _wrapm(zv, n2) if w2 else zv
+        okq = (h1 + yq >= 0) & (h1 + yq < s1) & (h2 + zq >= 0) & (h2 + zq + VZ <= s2)
+        mine = (ly >= GY) & (ly < GY + OTY) & (lz >= GZ) & (lz < GZ + OTZ) & iny & inz
+        vz_off = lz - lzv
+
+        def load_q0(x, live):
+            px, ok = plane(x)
+            ok = ok and live
+            vals = np.zeros((TY, TZ), dtype=np.float64)
+            if ok:
+                m = okq
+                if m.any():
+                    if (h2 + zq[m] + VZ > s2).any() or (h1 + yq[m] >= s1).any():
+                        errs.append(f"queue load outside plane {x}")
+                    vals[m] = u3[h0 + px, h1 + yq[m], h2 + zq[m] + vz_off[m]]
+            return vals
+
+        # level-0 halo ring of every buffer: LDS (row, column) and in-plane
+        # array coordinates of each fetched vector (fixed per workgroup)
+        rings = []
+        for b in bufs:
+            rc = np.array([gvec_rc(b, i) for i in range(b["gvecs"])], dtype=int).reshape(-1, 2)
+            rr, cc = rc[:, 0], rc[:, 1]
+            inside = (rr >= b["ry"]) & (rr < b["ry"] + TY) & (cc * VZ >= b["rzv"]) & (cc * VZ < b["rzv"] + TZ)
+            if inside.any():
+                errs.append("ring vector inside the tile interior")
+            cover = np.zeros((b["rows"], b["w"]), dtype=int)
+            np.add.at(cover, (rr, cc), 1)
+            cover[b["ry"]:b["ry"] + TY, b["sv"]:b["sv"] + TZ // VZ] += 1
+            if not (cover == 1).all():
+                errs.append("ring and interior do not tile the LDS plane")
+            gy, gz = y0 - b["ry"] + rr, z0 - b["rzv"] + cc * VZ
+            if w1:
+                gy = _wrapm(gy, n1)
+            if w2:
+                gz = _wrapm(gz, n2)
+            gy, gz = gy + h1, gz + h2
+            ok = (gy >= 0) & (gy < s1) & (gz >= 0) & (gz + VZ <= s2)
+            rings.append((rr, cc, gy, gz, ok))
+
+        def load_ring(x, live):
+            ring = []
+            for b, (rr, cc, gy, gz, ok) in zip(bufs, rings):
+                px, pok = plane(x + b["dx"])
+                v = np.zeros((len(rr), VZ))
+                if pok and live and ok.any():
+                    cols = gz[ok][:, None] + np.arange(VZ)[None, :]
+                    v[ok] = u3[h0 + px, gy[ok][:, None], cols]
+                ring.append(v)
+            return ring
+
+        s_first = xb + (TT - 1) * QLO
+        s_last = xe - 1 + (TT - 1) * D
+        q = [[np.zeros((TY, TZ)) for _ in range(QN)] for _ in range(TT)]
+        lds = [[np.zeros((b["rows"], b["cols"])) for b in bufs] for _ in range(TT)]
+        for i in range(QN - 1):
+            q[0][i] = load_q0(s_first + QLO + i, True)
+        pq = load_q0(s_first + QHI, True)
+        pb = load_ring(s_first, True)
+        for s in range(s_first, s_last + 1):
+            R = (s - s_first) % QN
+            q[0][_pmod(R - 1, QN)] = pq
+            for L in range(TT):
+                for k, b in enumerate(bufs):
+                    sl = _pmod(R - L * D + b["dx"] - QLO, QN)
+                    lds[L][k][b["ry"]:b["ry"] + TY, b["rzv"]:b["rzv"] + TZ] = q[L][sl]
+                    if L == 0:
+                        rr, cc = rings[k][0], rings[k][1]
+                        cols = cc[:, None] * VZ + np.arange(VZ)[None, :]
+                        lds[0][k][rr[:, None], cols] = pb[k]
+            pq = load_q0(s + 1 + QHI, s + 1 <= s_last)
+            pb = load_ring(s + 1, s + 1 <= s_last)
+            for L in range(TT, 0, -1):
+                RL = _pmod(R - (L - 1) * D, QN)
+                p = s - (L - 1) * D
+
+                def get(dx, dy, dz, L=L, RL=RL):
+                    if dy == 0 and dz == 0:
+                        if not QLO <= dx <= QHI:
+                            errs.append(f"queue read dx={dx} outside [{QLO}, {QHI}]")
+                            return np.zeros((TY, TZ))
+                        return q[L - 1][_pmod(dx - QLO + RL, QN)]
+                    if dx not in bidx:
+                        errs.append(f"no LDS buffer for dx={dx}")
+                        return np.zeros((TY, TZ))
+                    b = bufs[bidx[dx]]
+                    r0, c0 = b["ry"] + dy, b["rzv"] + dz
+                    if r0 < 0 or r0 + TY > b["rows"] or c0 < 0 or c0 + TZ > b["cols"]:
+                        errs.append(f"LDS read ({dx},{dy},{dz}) outside buffer")
+                        return np.zeros((TY, TZ))
+                    return lds[L - 1][bidx[dx]][r0:r0 + TY, c0:c0 + TZ]
+
+                val = np.broadcast_to(np.asarray(f(_Proxy(ndim, get), *([0] * ndim)), dtype=np.float64),
+                                      (TY, TZ))
+                if L == TT:
+                    if p >= xb:
+                        if not (0 <= p < n0):
+                            errs.append(f"store plane {p} outside the interior")
+                            continue
+                        m = mine
+                        res[h0 + p, h1 + y[m], h2 + z[m]] = val[m]
+                        written[h0 + p, h1 + y[m], h2 + z[m]] += 1
+                else:
+                    pin = w0 or (0 <= p < n0)
+                    sl = _pmod(R - (L - 1) * D - QLO, QN)
+                    if tmode == 0:
+                        q[L][sl] = np.where(pin & keep, val, 0.0)
+                    else:
+                        q[L][sl] = np.array(val, copy=True)
+    inner = tuple(slice(h, s - h) for h, s in zip((h0, h1, h2), u3.shape))
+    if written.max() > 1 or (not edges and not (written[inner] == 1).all()):
+        errs.append(f"interior cells written {written[inner].min()}..{written[inner].max()} times")
+    wmask = np.ones(u3.shape, dtype=bool)
+    wmask[inner] = False
+    if written[wmask].any():
+        errs.append("halo cells written")
+    return res.reshape(np.shape(u)), errs

@@ -196,9 +196,19 @@ class TestCompileAPI:
         ast.parse(r.code)
         assert r.temporal_steps == 3 and "TEMPORAL_STEPS = 3" in r.code
         assert "_KERNEL.launch_temporal([u_in], [u_out]" in r.code
-        assert "st::temporal_entry<P_tb3_" in r.hip_source
+        # register/LDS kernel at the vector width and at scalar width
+        assert r.hip_source.count("st::tblock_entry<P_tb3r") == 2
+        assert "st::temporal_entry" not in r.hip_source
         r2 = compile(S(HEAT2D), temporal_steps=2, temporal_mode="recompute")
         assert r2.halo_widths == (2, 16) and r2.stencil_halo == (1, 1)
+
+    def test_deep_wide_temporal_falls_back_to_lds_rings(self):
+        # 4 levels of a radius-4 star need a 12-cell ghost ring per side: no
+        # register/LDS geometry fits, the LDS-ring kernel is emitted instead
+        from cutile import workloads as w
+
+        r = compile(w.star25(), temporal_steps=4)
+        assert "st::temporal_entry<P_tb4_" in r.hip_source and "tblock" not in r.hip_source
 
     def test_multi_input_launcher_signature(self):
         r = compile(S("def wave(u, uold, i, j):\n    return 2.0 * u[i, j] - uold[i, j]\n"))

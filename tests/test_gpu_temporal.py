@@ -37,8 +37,10 @@ def test_zero_mode_equals_zero_halo_chain(oracle, which, T):
 
     fn = getattr(w, which)()
     r = compile(fn, temporal_steps=T)
-    assert r.temporal_steps == T and "temporal" in r.load_module()._VARIANTS
-    print(f"{which} T={T} tile {r.load_module()._VARIANTS['temporal']}", flush=True)
+    v = r.load_module()._VARIANTS
+    key = "tblock" if "tblock" in v else "temporal"
+    assert r.temporal_steps == T and key in v
+    print(f"{which} T={T} variant {v[key]}", flush=True)
     rng = np.random.default_rng(T)
     base = {2: (70, 150), 3: (23, 19, 45)}[r.ndim]
     shape = tuple(b + 2 * h for b, h in zip(base, r.halo_widths))
@@ -103,3 +105,23 @@ def test_graph_march_matches_eager(oracle, T):
     ref = oracle.time_march(u0, fn._fn, 3, ts.halo, 10, spec)
     assert np.array_equal(eager, ref)
     assert np.array_equal(graphed, ref)
+
+
+def test_graph_keeps_its_buffers_alive():
+    """The captured step loop writes its scratch array on every replay: the
+    stepper must hold it, or the caching allocator hands its memory to the
+    next tensor of that size and replays overwrite that tensor."""
+    import torch
+
+    from cutile import workloads as w
+    from cutile.runtime.timestep import TimeStepper
+
+    ts = TimeStepper(w.diffusion7(), w.periodic_spec(3), temporal_steps=1)
+    u0 = w.sinusoid_init((16, 20, 24), ts.halo, seed=5)
+    u = to_dev(u0)
+    ts.march(u, 4, graph=True, graph_steps=4)
+    torch.cuda.synchronize()
+    held = [torch.full_like(u, 7.0) for _ in range(4)]
+    ts._graph.launch()
+    torch.cuda.synchronize()
+    assert all(bool((t == 7.0).all()) for t in held)

@@ -26,6 +26,13 @@ def _cases():
         ("gray_scott_f32", lambda: compile_fused(list(w.gray_scott("float32")))),
         ("star25_f64_T2", lambda: compile(w.star25(), temporal_steps=2, temporal_mode="recompute")),
         ("heat2d_T4", lambda: compile(w.heat2d(), temporal_steps=4, temporal_mode="recompute")),
+        ("heat2d_T10", lambda: compile(w.heat2d(), temporal_steps=10)),
+        ("diffusion7_f64_T2", lambda: compile(w.diffusion7(), temporal_steps=2)),
+        ("diffusion7_f64_T4", lambda: compile(w.diffusion7(), temporal_steps=4)),
+        ("diffusion7_f32_T4", lambda: compile(w.diffusion7("float32"), temporal_steps=4)),
+        ("star25_f32_T2", lambda: compile(w.star25("float32"), temporal_steps=2)),
+        ("star25_f32_T4", lambda: compile(w.star25("float32"), temporal_steps=4)),
+        ("star25_f64_T4", lambda: compile(w.star25(), temporal_steps=4)),
         ("gray_scott_kahan_bf16", lambda: compile_fused(list(w.gray_scott_rates("float32")), update="kahan",
                                                         targets=["u", "v"], storage="bfloat16")),
         ("gray_scott_kahan_f32", lambda: compile_fused(list(w.gray_scott_rates("float32")), update="kahan",
@@ -37,7 +44,7 @@ def _cases():
 def test_default_variants_fit_registers(name, make):
     r = make()
     res = regcheck.code_object_resources(rt.compile_only(r.hip_source, compile_options("gfx950")))
-    kernels = {k: v for k, v in res.items() if "_sweep_" in k or "_temporal_" in k}
+    kernels = {k: v for k, v in res.items() if "_sweep_" in k or "_temporal_" in k or "_tblock_" in k}
     assert kernels, res
     for k, v in kernels.items():
         assert v.stack_bytes == 0, f"{k} spills ({v.stack_bytes} B scratch)"

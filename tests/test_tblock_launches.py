@@ -1,0 +1,126 @@
+"""The temporal-blocking launches the device tests and the benchmark make,
+replayed on the host model (CPU only).
+
+Each case compiles the stencil as the device path does, builds the launch
+arguments through ``StencilKernel.args`` (256 CUs, the MI355X count), and runs
+the model with the variant and chunk length the launch would use: values
+against the oracle for the test sizes, bounds of the edge workgroups for the
+benchmark sizes."""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import tblock_model as tm
+from cutile import compile
+from cutile import workloads as w
+from cutile.frontend.ir import find_func
+from cutile.lowering.hip_emitter import (plan_accesses, tblock_config, tblock_queue_range, tblock_symbol)
+from cutile.lowering.meta import extract_meta
+from cutile.runtime import kernel as K
+
+
+@pytest.fixture(autouse=True)
+def _mi355x(monkeypatch):
+    monkeypatch.setattr(K, "cu_count", lambda: 256)
+    monkeypatch.setattr(K, "current_arch", lambda: "gfx950")
+
+
+def _launch_geometry(r, shape, storage):
+    """(variant config, chunk length) of a temporal launch on arrays of ``shape``."""
+    mod = r.load_module()
+    kern = mod._KERNEL
+    dt = getattr(torch, K.STORAGE_TORCH[kern.storage])
+    a, b = torch.zeros(shape, dtype=dt), torch.zeros(shape, dtype=dt)
+    key = kern.temporal_key([a], [b], r.halo_widths)
+    assert key.startswith("tblock"), kern.variants.keys()
+    args, _ = kern.args([a], [b], r.halo_widths, key)
+    return kern, key, args
+
+
+def _cfg_for(fn, r, key, storage):
+    meta = extract_meta(find_func(fn._ir))
+    vz = r.load_module()._KERNEL.variants[key][3]
+    cfg = tblock_config(meta, r.temporal_steps, storage, storage if storage != "bfloat16" else "float32", vz)
+    assert tblock_symbol(meta, cfg) == r.load_module()._KERNEL.variants[key][0]
+    return meta, cfg
+
+
+DEVICE_CASES = [
+    # the shapes of tests/test_gpu_temporal.py and tests/test_gpu_parity.py
+    ("heat2d", 4, (70, 150), 0),
+    ("diffusion7", 2, (23, 19, 45), 0),
+    ("diffusion7", 3, (23, 19, 45), 0),
+    ("star25", 2, (23, 19, 45), 0),
+    ("star25", 2, (24, 20, 36), 1),
+    ("diffusion7", 4, (30, 18, 50), 1),
+    ("heat2d", 4, (100, 100), 0),
+    ("diffusion7", 2, (16, 20, 24), 1),
+    ("heat2d", 3, (33, 47), 0),
+    ("heat2d", 3, "validate", 0),       # CompileResult.validate: 64 + 2*T*h per axis, halo included
+    # TimeStepper on a periodic problem: blocks read around every axis
+    ("star25", 2, (24, 20, 36), "wrap"),
+    ("diffusion7", 4, (30, 18, 50), "wrap"),
+    ("diffusion7", 2, (16, 20, 24), "wrap"),
+]
+
+
+@pytest.mark.parametrize("which,T,n,tmode", DEVICE_CASES,
+                         ids=[f"{c[0]}-T{c[1]}-{c[2] if isinstance(c[2], str) else 'x'.join(map(str, c[2]))}-m{c[3]}"
+                              for c in DEVICE_CASES])
+def test_device_case_matches_oracle(which, T, n, tmode):
+    fn = getattr(w, which)()
+    r = compile(fn, temporal_steps=T, temporal_mode="zero" if tmode == 0 else "recompute")
+    wrap = 0
+    if tmode == "wrap":
+        tmode, wrap = 0, {1: 4, 2: 5, 3: 7}[r.ndim]
+    halo = r.halo_widths
+    if n == "validate":
+        n = tuple(64 + 2 * T * h - 2 * h for h in halo)
+    shape = tuple(a + 2 * h for a, h in zip(n, halo))
+    kern, key, args = _launch_geometry(r, shape, r.storage)
+    meta, cfg = _cfg_for(fn, r, key, r.storage)
+    u = np.random.default_rng(T).random(shape)
+    ndim = len(n)
+    if tmode == 0 and not wrap:
+        from test_tblock_model import _ref_zero
+        ref = _ref_zero(u, fn._fn, ndim, halo, T)
+    else:
+        spec = w.periodic_spec(ndim)
+        oracle.apply_boundary(u, spec, halo)
+        ref = oracle.time_march(u, fn._fn, ndim, halo, T, spec)
+    inner = oracle.interior(shape, halo)
+    if wrap:
+        stale = np.random.default_rng(T + 1).random(shape)   # the halo is stale: it must not be read
+        stale[inner] = u[inner]
+        u = stale
+    got, errs = tm.run(fn._fn, ndim, cfg, plan_accesses(meta), tblock_queue_range(meta), u, halo, args.xch,
+                       tmode=tmode, wrap=wrap)
+    assert not errs, errs[:5]
+    np.testing.assert_array_equal(got[inner], ref[inner])
+
+
+BENCH_CASES = [
+    ("star25", "float64", 2, (512, 512, 512), 7),
+    ("star25", "float32", 2, (512, 512, 512), 7),
+    ("diffusion7", "float64", 2, (512, 512, 512), 7),
+    ("diffusion7", "float64", 4, (512, 512, 512), 7),
+    ("heat2d", "float64", 4, (1024, 1024), 0),
+    ("heat2d", "float64", 10, (1024, 1024), 0),
+]
+
+
+@pytest.mark.parametrize("which,dtype,T,n,wrap", BENCH_CASES,
+                         ids=[f"{c[0]}-{c[1]}-T{c[2]}-w{c[4]}" for c in BENCH_CASES])
+def test_bench_launch_in_bounds(which, dtype, T, n, wrap):
+    fn = getattr(w, which)(dtype)
+    r = compile(fn, temporal_steps=T, temporal_mode="zero")
+    halo = r.halo_widths
+    shape = tuple(a + 2 * h for a, h in zip(n, halo))
+    kern, key, args = _launch_geometry(r, shape, r.storage)
+    meta, cfg = _cfg_for(fn, r, key, r.storage)
+    u = np.zeros(shape)
+    got, errs = tm.run(fn._fn, len(n), cfg, plan_accesses(meta), tblock_queue_range(meta), u, halo, args.xch,
+                       tmode=0, wrap=wrap, edges=True)
+    assert not errs, errs[:5]

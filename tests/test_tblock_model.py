@@ -1,0 +1,112 @@
+"""Host model of the register/LDS temporal-blocking kernel (CPU only).
+
+For the geometries the emitter picks, the model (``tblock_model.py``, the
+kernel's schedule step by step) must produce exactly the oracle's T-step
+results in all three semantics (zero halo between levels, recompute in a
+deep filled halo, periodic wrap) with no access outside its buffer, for
+several chunk lengths and ragged tile counts.  Only geometries that pass here
+run on a device."""
+
+import numpy as np
+import pytest
+
+import oracle
+import tblock_model as tm
+from cutile import workloads as w
+from cutile.frontend.decorator import stencil_from_source
+from cutile.frontend.ir import find_func
+from cutile.lowering.hip_emitter import plan_accesses, radius3, tblock_config, tblock_queue_range
+from cutile.lowering.meta import extract_meta
+
+
+def box27():
+    return stencil_from_source("""
+def box27(u, i, j, k):
+    return (0.3 * u[i, j, k] + 0.05 * (u[i - 1, j - 1, k] + u[i + 1, j + 1, k] + u[i - 1, j, k + 1]
+            + u[i + 1, j, k - 1] + u[i, j - 1, k - 1] + u[i, j + 1, k + 1]) + 0.01 * u[i + 1, j, k]
+            + 0.02 * u[i, j, k - 1] - 0.015 * u[i - 1, j + 1, k - 1])
+""")
+
+
+def line1d():
+    return stencil_from_source("""
+def line1d(u, i):
+    return 0.5 * u[i] + 0.2 * (u[i - 2] + u[i + 1]) + 0.1 * u[i + 2]
+""")
+
+
+def _setup(fn, T, storage, vz):
+    meta = extract_meta(find_func(fn._ir))
+    cfg = tblock_config(meta, T, storage, storage, vz)
+    assert cfg is not None
+    return meta, cfg, plan_accesses(meta), tblock_queue_range(meta)
+
+
+def _ref_zero(u, fn, ndim, halo, T):
+    """T chained sweeps; levels >= 1 pass through zero-halo arrays."""
+    cur = u
+    inner = oracle.interior(u.shape, halo)
+    for t in range(T):
+        nxt = oracle.apply_stencil(cur, fn, ndim, halo)
+        if t + 1 < T:
+            cur = np.zeros_like(nxt)
+            cur[inner] = nxt[inner]
+        else:
+            cur = nxt
+    return cur
+
+
+CASES = [
+    # name, stencil factory, T, storage for the geometry, vz, interior, halo
+    ("heat2d-T4-v2", w.heat2d, 4, "float64", 2, (37, 300), (4, 16)),
+    ("heat2d-T4-v1", w.heat2d, 4, "float64", 1, (37, 301), (4, 5)),
+    ("heat2d-T10-v2", w.heat2d, 10, "float64", 2, (23, 260), (10, 16)),
+    ("diff7-T2-v2", w.diffusion7, 2, "float64", 2, (9, 50, 70), (2, 2, 16)),
+    ("diff7-T4-v1", w.diffusion7, 4, "float64", 1, (9, 41, 37), (4, 4, 4)),
+    ("star25-T2-v2", w.star25, 2, "float64", 2, (10, 40, 36), (8, 8, 16)),
+    ("star25-T2-v1", w.star25, 2, "float64", 1, (10, 37, 35), (8, 8, 8)),
+    ("star25f-T2-v4", w.star25, 2, "float32", 4, (10, 56, 72), (8, 8, 32)),
+    ("box27-T2-v2", box27, 2, "float64", 2, (8, 20, 30), (2, 2, 16)),
+    ("box27-T3-v1", box27, 3, "float64", 1, (8, 21, 31), (3, 3, 3)),
+    ("line1d-T3-v1", line1d, 3, "float64", 1, (50,), (6,)),
+]
+
+
+@pytest.mark.parametrize("mode", ["zero", "recompute", "wrap"])
+@pytest.mark.parametrize("name,make,T,storage,vz,n,halo", CASES, ids=[c[0] for c in CASES])
+def test_model_matches_oracle(name, make, T, storage, vz, n, halo, mode):
+    fn = make()
+    ndim = len(n)
+    meta, cfg, plan, qr = _setup(fn, T, storage, vz)
+    rng = np.random.default_rng(len(name) + T)
+    shape = tuple(a + 2 * h for a, h in zip(n, halo))
+    u = rng.random(shape)
+    spec = w.periodic_spec(ndim)
+    if mode == "zero":
+        ref = _ref_zero(u, fn._fn, ndim, halo, T)
+        tmode, wrap = 0, 0
+    else:
+        up = u.copy()
+        oracle.apply_boundary(up, spec, halo)
+        ref = oracle.time_march(up, fn._fn, ndim, halo, T, spec)
+        if mode == "recompute":
+            u, tmode, wrap = up, 1, 0
+        else:
+            tmode, wrap = 0, 7
+    inner = oracle.interior(shape, halo)
+    for xch in sorted({3, 7, n[0]} if ndim > 1 else {1}):
+        got, errs = tm.run(fn._fn, ndim, cfg, plan, qr, u, halo, xch, tmode=tmode, wrap=wrap)
+        assert not errs, (xch, errs[:5])
+        np.testing.assert_array_equal(got[inner], ref[inner], err_msg=f"xch={xch}")
+
+
+@pytest.mark.parametrize("name,make,T,storage,vz,n,halo", CASES, ids=[c[0] for c in CASES])
+def test_geometry_invariants(name, make, T, storage, vz, n, halo):
+    fn = make()
+    meta, cfg, plan, (qlo, qhi) = _setup(fn, T, storage, vz)
+    rx, ry, rz = radius3(meta)
+    assert qlo <= 0 <= qhi and qhi - qlo >= 2 * rx if rx else True
+    assert cfg.gy >= (T - 1) * ry and cfg.gz >= (T - 1) * rz
+    assert cfg.gz % cfg.vz == 0 and cfg.tz % cfg.vz == 0 and cfg.otz % cfg.vz == 0
+    assert cfg.oty > 0 and cfg.otz > 0
+    assert cfg.kc * cfg.nt >= cfg.ty * cfg.tz // cfg.vz > (cfg.kc - 1) * cfg.nt
