@@ -157,34 +157,50 @@ def _timed(stream, fn, rt):
 
 
 def temporal_depths(interior_dev, n, steps=48):
-    """25-point star, float64 and float32, temporal depth 1, 2 and 4 (one
-    launch advances T steps; periodic halo T*4 deep refilled per launch),
-    ``steps`` steps as one captured hipGraph."""
+    """25-point star (the C2 workload) and 7-point diffusion, float64 and
+    float32, temporal depth 1, 2 and 4, periodic: ``steps`` steps as one
+    captured hipGraph.  At depth 1 every step is one wrapped sweep; deeper
+    blocks advance T steps per launch in registers and LDS, reading around
+    the periodic axes (no halo fills); where a depth's ghost ring does not
+    fit (25-point at depth 4) the LDS-ring kernel runs with a T*r-deep halo
+    refilled per launch.  Every depth's state after ``steps`` steps is
+    compared bit for bit with depth 1's."""
     import torch
 
     from cutile.runtime import rt
     from cutile.runtime.timestep import TimeStepper
-    from cutile.workloads import periodic_spec, star25
+    from cutile.workloads import diffusion7, periodic_spec, star25
 
     out = {}
-    for dt_name in ("float64", "float32"):
-        for T in (1, 2, 4):
-            key = f"star25_{dt_name}_T{T}"
-            st = TimeStepper(star25(dt_name), periodic_spec(3), temporal_steps=T)
-            u = st.empty_like_interior((n, n, n))
-            u[tuple(slice(h, h + n) for h in st.halo)] = interior_dev.to(u.dtype)
-            # capture the step loop once (JIT build, one replay), then time replays
-            st.march(u, steps, graph=True, graph_steps=steps)
-            torch.cuda.synchronize()
-            g = st._graph
-            ms = _timed(g.stream, g.launch, rt)
-            torch.cuda.synchronize()
-            out[key] = {"T": T, "dtype": dt_name, "steps": steps, "halo": list(st.halo),
-                        "ms_per_step": ms / steps, "gcell_updates_per_s": n ** 3 * steps / (ms * 1e-3) / 1e9,
-                        "alg_bytes_per_update": 2 * u.element_size() / T}
-            log(f"{key}: {ms / steps:.4f} ms/step, {out[key]['gcell_updates_per_s']:.1f} Gcell-updates/s")
-            del u, st, g
-            torch.cuda.empty_cache()
+    for sname, mk in (("star25", star25), ("diffusion7", diffusion7)):
+        for dt_name in ("float64", "float32"):
+            first = None
+            for T in (1, 2, 4):
+                key = f"{sname}_{dt_name}_T{T}"
+                st = TimeStepper(mk(dt_name), periodic_spec(3), temporal_steps=T)
+                u = st.empty_like_interior((n, n, n))
+                inner = tuple(slice(h, h + n) for h in st.halo)
+                u[inner] = interior_dev.to(u.dtype)
+                # capture the step loop once (JIT build, one replay), then time replays
+                st.march(u, steps, graph=True, graph_steps=steps)
+                torch.cuda.synchronize()
+                state = u[inner].clone()
+                if first is None:
+                    first = state
+                same = bool(torch.equal(state, first))
+                g = st._graph
+                ms = _timed(g.stream, g.launch, rt)
+                torch.cuda.synchronize()
+                kind = "sweep" if st._block is None else (
+                    "registers+LDS" if "tblock" in st._block.variants else "LDS rings")
+                out[key] = {"T": T, "dtype": dt_name, "steps": steps, "halo": list(st.halo), "kernel": kind,
+                            "ms_per_step": ms / steps, "gcell_updates_per_s": n ** 3 * steps / (ms * 1e-3) / 1e9,
+                            "alg_bytes_per_update": 2 * u.element_size() / T, "state_equals_T1": same}
+                log(f"{key}: {ms / steps:.4f} ms/step, {out[key]['gcell_updates_per_s']:.1f} Gcell-updates/s "
+                    f"({kind}; state after {steps} steps equal to T=1: {same})")
+                del u, st, g, state
+                torch.cuda.empty_cache()
+            del first
     return out
 
 
@@ -500,6 +516,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--core", action="store_true",
                     help="extras limited to the other 3-D sweeps and the copy reference (for profiling)")
+    ap.add_argument("--only", choices=["temporal", "heat2d", "gray_scott", "batched", "serving"],
+                    help="the headline plus this one extra section (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py drives one GPU: this build runs every workload on a single MI355X "
@@ -525,6 +543,13 @@ def main():
 
     head = measure("star25_fp64", star25("float64"), interior, n, args.steps, args.warmup, "f64", spec)
     extra = {}
+    if args.only:
+        args.no_extra = True
+        sect = {"temporal": ("temporal_depths_c2", lambda: temporal_depths(interior, n)),
+                "heat2d": ("heat2d_c1", heat2d_c1), "gray_scott": ("gray_scott_c3", gray_scott_c3),
+                "batched": ("batched_wave_c4", batched_wave_c4), "serving": ("serving_c5", serving_c5)}
+        key, fn = sect[args.only]
+        extra[key] = fn()
     if not args.no_extra:
         extra["diffusion7_fp64"] = measure("diffusion7_fp64", diffusion7("float64"), interior, n, args.steps,
                                            args.warmup, "f64", spec)

@@ -43,7 +43,7 @@ int fail(int code, const char* fmt, ...) {
       return fail((int)e_, "%s failed: %s", #expr, hipGetErrorString(e_));       \
   } while (0)
 
-uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull) {
+uint64_t fnv1a(const void* data, size_t n, uint64_t h = 14695981039346656037ull) {
   const unsigned char* p = static_cast<const unsigned char*>(data);
   for (size_t i = 0; i < n; ++i) {
     h ^= p[i];
@@ -353,8 +353,11 @@ int st_compile(const char* src, const char* name, const char* const* opts, int n
   if (!src || !module) return fail(ST_E_INVALID, "null argument");
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
+  // key: the source and every option except include paths (the same tree
+  // may sit at another path; header contents are keyed through a -D option)
   uint64_t h = fnv1a(src, strlen(src));
   for (int i = 0; i < nopts; ++i) {
+    if (strncmp(opts[i], "-I", 2) == 0) continue;
     h = fnv1a(opts[i], strlen(opts[i]), h);
     h = fnv1a("\x1f", 1, h);
   }

@@ -176,13 +176,15 @@ def tblock_estimate(meta: StencilMeta, cfg: TBlockConfig, storage: str, compute:
 
 
 def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int,
-                  budget: int = 160 * 1024, max_otz: int = 256) -> "TBlockConfig | None":
+                  budget: int = 160 * 1024, max_otz: int = 256, force: "tuple | None" = None) -> "TBlockConfig | None":
     """Cheapest register/LDS temporal-blocking geometry for ``T`` levels at
     vector width ``vz``: fewest lane-cells evaluated per output cell (ghost
     ring and idle lanes included) whose queues fit the VGPR budget of its
     workgroup size and whose level planes fit the LDS.  None when nothing
     fits (deep ghosts of wide stencils); the caller falls back to the
-    LDS-ring kernel."""
+    LDS-ring kernel.  ``force`` = (output rows, output columns, threads)
+    restricts the search to that shape (a tuning hint; it must satisfy the
+    same limits)."""
     if meta.num_inputs != 1 or meta.num_exprs != 1 or meta.update or T < 2:
         return None
     rx, ry, rz = radius3(meta)
@@ -192,12 +194,17 @@ def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
     best = None
     otzs = [t for t in (16, 32, 64, 128, 256, 512, 1024) if t <= max_otz and t % vz == 0]
     otys = range(1, 65) if nd == 3 else (1,)
+    nts = range(64, 769, 64)     # 16-wave workgroups measured slower (barriers across 16 waves)
+    if force is not None:
+        otys, otzs, nts = (int(force[0]),), (int(force[1]),), (int(force[2]),)
+        if otzs[0] % vz or (nd < 3 and otys[0] != 1):
+            return None
     for otz in otzs:
         tz = otz + 2 * gz
         for oty in otys:
             ty = oty + 2 * gy
             cells = ty * (tz // vz)
-            for nt in range(64, 1025, 64):
+            for nt in nts:
                 kc = -(-cells // nt)
                 if kc > 4:
                     continue
@@ -207,11 +214,18 @@ def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
                 est = tblock_estimate(meta, cfg, storage, compute)
                 if est["lds"] > budget or est["vgpr"] > 0.85 * _vgpr_limit(nt):
                     continue
-                cost = kc * nt * vz / (oty * otz)
+                # lane-cells evaluated per output cell (ghost ring and idle
+                # lanes), with the penalties measured on MI355X for narrow
+                # output columns (halo rows per LDS row, short contiguous
+                # runs) and for several cells per thread (register pressure)
+                cost = kc * nt * vz / (oty * otz) * (1 + 16 / otz) * (1 + 0.05 * (kc - 1))
                 key = (round(cost, 3), -(oty * otz), nt)
                 if best is None or key < best[0]:
                     best = (key, cfg)
-    return best[1] if best else None
+    # more than ~6 lane-cells per output cell loses to T separate sweeps
+    if best is None or best[0][0] > 6.0:
+        return None
+    return best[1]
 
 
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,

@@ -125,8 +125,10 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
         # register/LDS kernel at the vector width and at scalar width (arrays
         # whose rows are not vector-aligned); the LDS-ring kernel when the
         # ghost ring of T levels does not fit the registers or the LDS
+        force = (kernel_hints or {}).get("tblock_tile")
         for vz in dict.fromkeys((min(vec.vz, 4), 1)):
-            tb = tblock_config(meta, int(temporal_steps), storage_t, compute_t, vz)
+            tb = tblock_config(meta, int(temporal_steps), storage_t, compute_t, vz,
+                               force=force if vz > 1 or vec.vz == 1 else None)
             if tb is None:
                 tblocks = []
                 break

@@ -151,12 +151,11 @@ class BatchedStencil:
     def _function(self):
         import torch
 
-        from cutile.runtime.kernel import cache_dir, compile_options, current_arch
+        from cutile.runtime.kernel import load_or_compile
 
         dev = torch.cuda.current_device()
         if dev not in self._fn:
-            mod = rt.compile_module(self.source, compile_options(current_arch()), cache_dir(),
-                                    name=f"{self.symbol}.hip")
+            mod = load_or_compile(self.source, f"{self.symbol}.hip")
             self._fn[dev] = rt.get_function(mod, self.symbol)
         return self._fn[dev]
 

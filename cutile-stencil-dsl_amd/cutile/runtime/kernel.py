@@ -80,12 +80,59 @@ def cache_dir() -> str:
     return d
 
 
+_hdr_digest: list = []
+
+
+def header_digest() -> str:
+    """Hash of the device headers the generated sources include: part of the
+    options, so cached code objects are rebuilt when a header changes."""
+    if not _hdr_digest:
+        names = sorted(n for n in os.listdir(rt.CSRC_DIR) if n.endswith(".h"))
+        blob = b"".join(n.encode() + b"\0" + open(os.path.join(rt.CSRC_DIR, n), "rb").read() for n in names)
+        _hdr_digest.append(rt.cache_key(blob.decode("latin-1"), []))
+    return _hdr_digest[0]
+
+
 def compile_options(arch: Optional[str] = None) -> list:
     arch = arch or "gfx950"
     return [f"--offload-arch={arch}", "-O3", "-std=c++17", "-ffp-contract=off",
             "-fno-gpu-flush-denormals-to-zero",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            f"-I{rt.CSRC_DIR}"]
+            f"-DST_HDR_DIGEST=0x{header_digest()}ull", f"-I{rt.CSRC_DIR}"]
+
+
+def aot_dir() -> str:
+    """Code objects written ahead of time (``precompile``; read-only at run
+    time): ``$CUTILE_AMD_AOT`` or the package's ``_aot`` folder."""
+    return os.environ.get("CUTILE_AMD_AOT", os.path.join(os.path.dirname(os.path.dirname(__file__)), "_aot"))
+
+
+def precompile(source: str, arch: str = "gfx950", directory: Optional[str] = None) -> str:
+    """Build ``source`` for ``arch`` without a device and store the code
+    object where a later launch on such a device looks first.  Returns its
+    path."""
+    d = directory or aot_dir()
+    os.makedirs(d, exist_ok=True)
+    opts = compile_options(arch)
+    path = os.path.join(d, rt.cache_key(source, opts) + ".hsaco")
+    if not os.path.exists(path):
+        code = rt.compile_only(source, opts)
+        tmp = f"{path}.tmp{os.getpid()}"
+        with open(tmp, "wb") as f:
+            f.write(code)
+        os.replace(tmp, path)
+    return path
+
+
+def load_or_compile(source: str, name: str):
+    """Module for ``source`` on the current device: a code object written
+    ahead of time when there is one, else a JIT build (cached on disk)."""
+    opts = compile_options(current_arch())
+    path = os.path.join(aot_dir(), rt.cache_key(source, opts) + ".hsaco")
+    if os.path.exists(path):
+        with open(path, "rb") as f:
+            return rt.load_code(f.read())
+    return rt.compile_module(source, opts, cache_dir(), name=name)
 
 
 _arch_cache: dict = {}
@@ -126,14 +173,38 @@ def chunk_length(n0: int, tiles: int, default: int, queue_depth: int, per_cu: in
     return max(1, min(x, n0))
 
 
-def tblock_chunk_length(n0: int, tiles: int, default: int, depth: int) -> int:
-    """Planes per workgroup of a temporal-blocking launch: about two
-    workgroups per CU, at least ``depth`` planes (the warm-up of the skewed
-    levels costs ``depth`` steps per chunk, so shorter chunks would spend
-    most of their time warming up)."""
-    chunks = max(1, -(-(2 * cu_count()) // max(1, tiles)))
-    x = min(max(1, default), -(-n0 // chunks))
-    return max(1, min(max(x, depth), n0))
+def tblock_chunk_length(n0: int, tiles: int, default: int, depth: int, slots: int) -> int:
+    """Planes per workgroup of a temporal-blocking launch.
+
+    ``slots`` workgroups run at once (CUs x workgroups per CU); a launch of
+    ``tiles * c`` workgroups takes ``ceil(tiles * c / slots)`` rounds of
+    ``ceil(n0 / c) + depth`` steps each (every chunk warms the skewed levels
+    up over ``depth`` steps).  The chunk count c minimising that product is
+    taken: a partly filled last round and short chunks both cost time (on
+    MI355X a 2.25-round launch ran 25 % slower than a 2-round one of the same
+    kernel).  Chunks stay within the compiled chunk length ``default``."""
+    best = None
+    for c in range(max(1, -(-n0 // max(1, default))), n0 + 1):
+        x = -(-n0 // c)
+        t = -(-(tiles * c) // max(1, slots)) * (x + depth)
+        if best is None or t < best[0]:
+            best = (t, x)
+        if tiles * c > 64 * max(1, slots):
+            break
+    return max(1, best[1])
+
+
+def workgroups_per_cu(res, threads: int) -> int:
+    """Workgroups of ``threads`` threads one CU holds at once, from a
+    kernel's registers and LDS (4 SIMDs of 512 VGPRs per lane, 160 KiB LDS,
+    at most 8 waves per SIMD)."""
+    waves = -(-threads // 64)
+    per_simd = -(-waves // 4)
+    regs = max(8, -(-max(1, res.num_regs) // 8) * 8)
+    by_regs = 512 // (regs * per_simd) if regs * per_simd <= 512 else 0
+    by_waves = 8 // per_simd
+    by_lds = (160 * 1024) // res.shared_bytes if res.shared_bytes > 0 else by_waves
+    return max(1, min(by_regs, by_waves, by_lds))
 
 
 def xcd_count(arch: str) -> int:
@@ -161,6 +232,7 @@ class StencilKernel:
         self.queue_depth = queue_depth
         self.temporal_steps = temporal_steps
         self.tblock_depth = tblock_depth
+        self._tb_chunks: dict = {}
         self._modules: dict = {}
         self._fns: dict = {}
         self._lock = threading.Lock()
@@ -175,8 +247,7 @@ class StencilKernel:
                 mod = self._modules.get(dev)
                 if mod is None:
                     rt.check(rt.lib().st_init(dev), "st_init")
-                    mod = rt.compile_module(self.source, compile_options(current_arch()), cache_dir(),
-                                            name=f"{self.name}.hip")
+                    mod = load_or_compile(self.source, f"{self.name}.hip")
                     self._modules[dev] = mod
                 fn = rt.get_function(mod, self.variants[key][0])
                 if key.startswith("tblock"):
@@ -262,7 +333,14 @@ class StencilKernel:
         a.nty = -(-n[1] // ty)
         if key.startswith("tblock"):
             # T levels skewed by a queue length each: depth planes of warm-up per chunk
-            x = xch if xch else tblock_chunk_length(n[0], a.ntz * a.nty, self.xch, self.tblock_depth)
+            if not xch:
+                ck = (key, n[0], a.ntz * a.nty)
+                xch = self._tb_chunks.get(ck)
+                if xch is None:
+                    slots = cu_count() * workgroups_per_cu(self.resources(key), nt)
+                    xch = tblock_chunk_length(n[0], a.ntz * a.nty, self.xch, self.tblock_depth, slots)
+                    self._tb_chunks[ck] = xch
+            x = xch
         else:
             # the LDS-ring temporal kernel warms up (queue_depth - 1) * T planes per chunk
             depth = self.queue_depth if key != "temporal" else (self.queue_depth - 1) * self.temporal_steps + 1

@@ -195,6 +195,36 @@ def compile_module(src: str, opts: Sequence[str], cache_dir: Optional[str], name
     return mod.value
 
 
+def load_code(code: bytes):
+    """Load a code object (e.g. one written ahead of time) as a module on the
+    current device."""
+    mod = ctypes.c_void_p()
+    buf = ctypes.create_string_buffer(code, len(code))
+    check(lib().st_load_code(buf, len(code), ctypes.byref(mod)), "st_load_code")
+    return mod.value
+
+
+def cache_key(src: str, opts: Sequence[str]) -> str:
+    """File stem under which ``st_compile`` caches the code object of ``src``
+    built with ``opts`` (FNV-1a over the source and the options other than
+    include paths, each followed by 0x1f)."""
+    h = 0xcbf29ce484222325
+    mask = (1 << 64) - 1
+
+    def feed(data: bytes, h: int) -> int:
+        for c in data:
+            h = ((h ^ c) * 0x100000001b3) & mask
+        return h
+
+    h = feed(src.encode(), h)
+    for o in opts:
+        if o.startswith("-I"):
+            continue
+        h = feed(o.encode(), h)
+        h = feed(b"\x1f", h)
+    return f"{h:016x}"
+
+
 def get_function(module, kernel: str):
     fn = ctypes.c_void_p()
     check(lib().st_get_function(module, kernel.encode(), ctypes.byref(fn)), f"st_get_function({kernel})")

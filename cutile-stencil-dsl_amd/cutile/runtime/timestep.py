@@ -71,7 +71,7 @@ class TimeStepper:
     by ``apply_boundary``."""
 
     def __init__(self, stencil_fn, boundary, temporal_steps: int = 1, storage: Optional[str] = None,
-                 tile_sizes=None, kernel_hints: Optional[dict] = None):
+                 tile_sizes=None, kernel_hints: Optional[dict] = None, block_hints: Optional[dict] = None):
         from cutile.runtime.launcher import compile as _compile
 
         self.fn = stencil_fn
@@ -86,7 +86,8 @@ class TimeStepper:
         self.single = _compile(stencil_fn, temporal_blocking=False, storage=storage, tile_sizes=tile_sizes,
                                kernel_hints=kernel_hints)
         if self.T > 1:
-            self.block = _compile(stencil_fn, temporal_steps=self.T, temporal_mode=self.mode, storage=storage)
+            self.block = _compile(stencil_fn, temporal_steps=self.T, temporal_mode=self.mode, storage=storage,
+                                  kernel_hints=block_hints)
             self.halo = self.block.halo_widths
         else:
             self.block = None

@@ -95,7 +95,9 @@ void st_free(void* p);
 
 /* Compile (or fetch from the in-process / on-disk cache under `cache_dir`,
  * may be NULL) and load a module on the current device.  The handle stays
- * valid until st_unload. */
+ * valid until st_unload.  The cache key is the FNV-1a hash of the source and
+ * of every option except "-I..." include paths, each option followed by the
+ * byte 0x1f; the file is `<cache_dir>/<16 hex digits>.hsaco`. */
 int st_compile(const char* src, const char* name, const char* const* opts, int nopts,
                const char* cache_dir, void** module);
 int st_load_code(const void* code, size_t size, void** module);

@@ -23,8 +23,18 @@ from cutile.runtime import kernel as K
 
 @pytest.fixture(autouse=True)
 def _mi355x(monkeypatch):
+    from cutile.runtime import regcheck, rt
+
     monkeypatch.setattr(K, "cu_count", lambda: 256)
     monkeypatch.setattr(K, "current_arch", lambda: "gfx950")
+
+    def resources(self, key="vec"):
+        # what the device would report, from the code object's notes
+        res = regcheck.code_object_resources(rt.compile_only(self.source, K.compile_options("gfx950")))
+        v = res[self.variants[key][0]]
+        return rt.Resources(num_regs=v.registers, local_bytes=v.stack_bytes, shared_bytes=v.lds_bytes)
+
+    monkeypatch.setattr(K.StencilKernel, "resources", resources)
 
 
 def _launch_geometry(r, shape, storage):
