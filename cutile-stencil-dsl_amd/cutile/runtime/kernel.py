@@ -66,7 +66,11 @@ def map3_shape(ndim: int, shape: Sequence[int]) -> tuple:
 
 
 def default_stream():
+    """Raw handle of torch's current stream on the current device."""
     torch = _torch()
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        return raw(torch.cuda.current_device())
     return torch.cuda.current_stream().cuda_stream
 
 
@@ -160,16 +164,22 @@ def chunk_length(n0: int, tiles: int, default: int, queue_depth: int, per_cu: in
 
     The grid gets about ``per_cu`` workgroups per CU: ``ceil(per_cu * CUs /
     tiles)`` chunks of ``ceil(n0 / chunks)`` planes, capped at the compiled
-    chunk ``default`` and kept at 4 queue refills or more so the
-    register-queue prologue (``queue_depth - 1`` planes) stays a small
-    fraction of the chunk.  On MI355X one long-streaming workgroup per CU
-    beat shorter chunks and several resident workgroups per CU for the 3-D
-    sweeps at 512^3 (fewer prologue re-reads, neighbouring tiles advancing in
-    step); the one-row tiles of 2-D sweeps ran 3-8 % faster at two per CU
-    (fused Gray-Scott at 4096^2, all four precisions)."""
-    chunks = max(1, -(-(max(1, per_cu) * cu_count()) // max(1, tiles)))
+    chunk ``default``.  Chunks are kept at 4 queue refills or more (the
+    register-queue prologue, ``queue_depth - 1`` planes, then stays a small
+    fraction of the chunk) when the grid still fills the CUs that way; a
+    small domain gets shorter chunks instead, since there the number of
+    workgroups, not the prologue's re-reads, sets the time.  On MI355X one
+    long-streaming workgroup per CU beat shorter chunks and several resident
+    workgroups per CU for the 3-D sweeps at 512^3 (fewer prologue re-reads,
+    neighbouring tiles advancing in step); the one-row tiles of 2-D sweeps
+    ran 3-8 % faster at two per CU (fused Gray-Scott at 4096^2, all four
+    precisions)."""
+    slots = max(1, per_cu) * cu_count()
+    chunks = max(1, -(-slots // max(1, tiles)))
     x = min(max(1, default), -(-n0 // chunks))
-    x = max(x, 4 * max(1, queue_depth - 1))
+    floor = 4 * max(1, queue_depth - 1)
+    if x < floor and max(1, tiles) * -(-n0 // floor) >= slots:
+        x = floor
     return max(1, min(x, n0))
 
 
@@ -233,6 +243,7 @@ class StencilKernel:
         self.temporal_steps = temporal_steps
         self.tblock_depth = tblock_depth
         self._tb_chunks: dict = {}
+        self._launches: dict = {}
         self._modules: dict = {}
         self._fns: dict = {}
         self._lock = threading.Lock()
@@ -405,15 +416,26 @@ class StencilKernel:
         reads periodically (past the interior it wraps around instead of
         reading the halo), so no halo fill is needed between steps; the
         output's halo is not written."""
-        if validate:
-            self.check_arrays(inputs, outputs)
-        key = self.pick(inputs, outputs, halo)
-        args, nblocks = self.args(inputs, outputs, halo, key, xch)
-        if args is None:
-            return
-        if periodic_axes:
-            args.wrap = self.wrap_mask(periodic_axes, args)
-        fn = self._function(key)
+        # repeated launches on the same arrays (a time loop) reuse the
+        # validated, packed arguments: the signature holds everything the
+        # checks and the arguments depend on
+        sig = (tuple((t.data_ptr(), tuple(t.shape), t.stride(), t.dtype, t.get_device())
+                     for t in (*inputs, *outputs)), tuple(halo), xch, tuple(periodic_axes))
+        hit = self._launches.get(sig)
+        if hit is None:
+            if validate:
+                self.check_arrays(inputs, outputs)
+            key = self.pick(inputs, outputs, halo)
+            args, nblocks = self.args(inputs, outputs, halo, key, xch)
+            if args is None:
+                return
+            if periodic_axes:
+                args.wrap = self.wrap_mask(periodic_axes, args)
+            hit = (self._function(key), nblocks, self.variants[key][4], args)
+            if len(self._launches) >= 256:
+                self._launches.clear()
+            self._launches[sig] = hit
+        fn, nblocks, nt, args = hit
         if stream is None:
             stream = default_stream()
-        rt.launch(fn, (nblocks, 1, 1), (self.variants[key][4], 1, 1), args, stream)
+        rt.launch(fn, (nblocks, 1, 1), (nt, 1, 1), args, stream)

@@ -1,0 +1,77 @@
+"""The sweep launches of the application drivers and the benchmark suite,
+replayed on the host bounds model (CPU only).
+
+Each case compiles the stencil as the driver does, builds the launch
+arguments through ``StencilKernel.args`` (256 CUs, the MI355X count; the
+variant the launch picks) and checks every global and LDS access of the
+edge workgroups with ``kernel_model.check_launch`` before any of them runs
+on a device."""
+
+import pytest
+import torch
+
+import kernel_model as km
+from cutile import apps, compile, compile_fused, suite
+from cutile.frontend.ir import find_func
+from cutile.lowering.hip_emitter import KernelConfig, plan_accesses
+from cutile.lowering.meta import access_table, extract_fused_meta, extract_meta
+from cutile.runtime import kernel as K
+
+
+@pytest.fixture(autouse=True)
+def _mi355x(monkeypatch):
+    monkeypatch.setattr(K, "cu_count", lambda: 256)
+    monkeypatch.setattr(K, "current_arch", lambda: "gfx950")
+
+
+def _check(r, meta, shape):
+    kern = r.load_module()._KERNEL
+    dt = getattr(torch, K.STORAGE_TORCH[kern.storage])
+    ins = [torch.zeros(shape, dtype=dt) for _ in range(kern.num_inputs)]
+    outs = [torch.zeros(shape, dtype=dt) for _ in range(kern.num_outputs)]
+    key = kern.pick(ins, outs, r.halo_widths)
+    a, nblocks = kern.args(ins, outs, r.halo_widths, key)
+    _, ty, tz, vz, nt = kern.variants[key]
+    cfg = KernelConfig(ty, tz, vz, nt)
+    errs = km.check_launch(cfg, plan_accesses(meta), (a.s0, a.s1, a.s2), (a.h0, a.h1, a.h2), (a.n0, a.n1, a.n2),
+                           a.xch, a.yslab, accesses=access_table(meta), wrap=a.wrap)
+    assert not errs, errs[:5]
+    assert nblocks == a.ntz * a.nty * a.nch
+
+
+def _single(fn):
+    return extract_meta(find_func(fn._ir))
+
+
+APP_CASES = [
+    ("heat_1d", lambda: apps.heat_1d_stencil(), (128,)),
+    ("advection", lambda: apps.advection_stencil(), (200,)),
+    ("fdtd_E", lambda: apps.fdtd_stencils()[0], (500,)),
+    ("fdtd_H", lambda: apps.fdtd_stencils()[1], (500,)),
+    ("wave_2d", lambda: apps.wave_2d_stencil(), (64, 64)),
+    ("laplacian_3d", lambda: apps.laplacian_3d_stencil(), (16, 16, 16)),
+    ("heat_2d", lambda: apps.heat_2d_stencil(), (128, 128)),
+]
+
+
+@pytest.mark.parametrize("name,make,n", APP_CASES, ids=[c[0] for c in APP_CASES])
+def test_app_sweeps_in_bounds(name, make, n):
+    fn = make()
+    r = compile(fn, temporal_blocking=False)
+    _check(r, _single(fn), tuple(a + 2 * h for a, h in zip(n, r.halo_widths)))
+
+
+@pytest.mark.parametrize("which,n", [("shallow_water", 32), ("gray_scott", 64)])
+def test_app_fused_sweeps_in_bounds(which, n):
+    fns = list(apps.shallow_water_stencils() if which == "shallow_water" else apps.gray_scott_stencils())
+    r = compile_fused(fns)
+    meta = extract_fused_meta([f._ir for f in fns])
+    _check(r, meta, tuple(n + 2 * h for h in r.halo_widths))
+
+
+@pytest.mark.parametrize("name", list(suite.DOMAINS))
+def test_suite_sweeps_in_bounds(name):
+    fn, nd, order = suite.stencils()[name]
+    r = compile(fn, temporal_blocking=False)
+    for domain in suite.DOMAINS[name]:
+        _check(r, _single(fn), tuple(d + 2 * h for d, h in zip(domain, r.halo_widths)))
