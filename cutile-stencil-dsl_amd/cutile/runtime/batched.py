@@ -121,7 +121,7 @@ class BatchedStencil:
             self.cfgs.append(cfg)
             parts.append(emit_traits(meta, cfg, self.storage, self.compute, f"b{k}", partial=True))
             cases.append(f"      case {k}: st::sweep_tile<P_b{k}>(a, tz, ty, ch); break;")
-        self.symbol = f"{self.name}_batched"
+        self.symbol = f"{self.name}_{self.storage}_batched"
         # The work-list is implicit: instance k owns items first[k] ..
         # first[k+1]-1, decoded into (tile z, tile y, chunk) from its own
         # tile counts; a workgroup finds the instance of item i by binary

@@ -313,7 +313,11 @@ class StencilServer:
     def _class(self, job: Job) -> _Class:
         key = (job.kind, job.dtype)
         if key not in self.classes:
-            self.classes[key] = _Class(*key)
+            import torch
+
+            cls = _Class(*key)
+            cls.stream = torch.cuda.Stream()
+            self.classes[key] = cls
         return self.classes[key]
 
     def job_bytes(self, job: Job) -> int:
@@ -367,7 +371,8 @@ class StencilServer:
             while waiting and self.reserved + self.job_bytes(waiting[0]) <= self.budget:
                 j = waiting.pop(0)
                 cls = self._class(j)
-                cls.allocate(j)
+                with torch.cuda.stream(cls.stream):       # buffers and initial fields on the class's stream
+                    cls.allocate(j)
                 j.admitted = now()
                 self.reserved += j.nbytes
                 self.peak_reserved = max(self.peak_reserved, self.reserved)
@@ -387,17 +392,18 @@ class StencilServer:
             tr = time.perf_counter()
             for cls in changed:
                 cls.tick = 0
-                cls.rebuild()
+                with torch.cuda.stream(cls.stream):       # work-list uploads on the class's stream
+                    cls.rebuild()
             host["rebuild"] += time.perf_counter() - tr
-            # one tick: every class advances its jobs by one step
+            # one tick: every class advances its jobs by one step.  A class's
+            # buffers, uploads and launches all live on its own stream, so the
+            # classes never wait for each other: a class's next step starts
+            # as soon as its previous one ends, whatever the other classes
+            # are doing.  The main stream joins them only to mark the tick
+            # for retirement (and the host's in-flight limit).
             tl = time.perf_counter()
-            # the classes' launches run concurrently on their own streams (so
-            # one class's tail overlaps another's work), joined every tick
             busy = [c for c in self.classes.values() if c.jobs]
             for cls in busy:
-                if cls.stream is None:
-                    cls.stream = torch.cuda.Stream()
-                cls.stream.wait_stream(self.main)
                 cls.step(cls.stream.cuda_stream)
             for cls in busy:
                 self.main.wait_stream(cls.stream)
@@ -412,7 +418,8 @@ class StencilServer:
                 for cls in {self._class(j) for j in done_now}:
                     cls.jobs = [j for j in cls.jobs if j.done < j.steps]
                     cls.tick = 0
-                    cls.rebuild()
+                    with torch.cuda.stream(cls.stream):
+                        cls.rebuild()
                 host["rebuild"] += time.perf_counter() - tr
             ticks.append((ev, done_now))
             while len(ticks) > self.inflight:
