@@ -444,7 +444,7 @@ class StencilServer:
 
 
 def main(argv=None) -> int:
-    """``python -m cutile.serving [--rate R] [--seconds S] [--seed N]``: the
+    """``python -m cutile.serving [--rate R] [--seconds S] [--seed N] [--inflight K]``: the
     serving configuration (Poisson arrivals, mixed jobs) on the current
     device; prints one JSON line of latency and throughput figures."""
     import argparse
@@ -456,9 +456,11 @@ def main(argv=None) -> int:
     ap.add_argument("--seconds", type=float, default=60.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--budget-gib", type=float, default=None)
+    ap.add_argument("--inflight", type=int, default=3, help="ticks queued on the device ahead of the host")
     args = ap.parse_args(argv)
     jobs = generate_jobs(args.rate, args.seconds, args.seed)
-    srv = StencilServer(hbm_budget=None if args.budget_gib is None else int(args.budget_gib * 2**30))
+    srv = StencilServer(hbm_budget=None if args.budget_gib is None else int(args.budget_gib * 2**30),
+                        inflight=args.inflight)
     warm = generate_jobs(50.0, 0.2, args.seed + 1)   # build every class's kernels before the clock starts
     for j in warm:
         j.steps = 2
