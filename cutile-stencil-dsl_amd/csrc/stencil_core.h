@@ -79,6 +79,31 @@ struct SweepArgs {
   int yslab;        // tile order: 0 = XCD gets an x-slab (z, y, x-chunk), 1 = a y-slab (z, x-chunk, y)
   int wrap;         // bit d set: axis d is periodic; reads past the interior wrap around
                     // (the halo along it is neither read nor written)
+  int nby, nbz;     // brick-major arrays: bricks along axes 1 and 2 (unused for flat arrays)
+};
+
+// Address map of an array: flat C-contiguous (B0 == 0), or brick-major with
+// B0 x B1 x B2 bricks (nb0 x nby x nbz bricks in C order, each brick's cells
+// contiguous and row-major).  X, Y, Z are array coordinates (halo included).
+// base(X): element offset of plane X's first cell; ioff(Y, Z): offset of cell
+// (Y, Z) from there; bytes(X): bytes from base(X) to the end of its plane
+// (flat) or of its slab of bricks (brick-major).  A VZ-vector never crosses a
+// brick when B2 % VZ == 0 and Z % VZ == 0.
+template <int B0, int B1, int B2> struct Layout {
+  i64 plane;
+  int s2, nby, nbz;
+  ST_HD i64 base(int X) const {
+    if constexpr (B0 == 0) return (i64)X * plane;
+    else return (i64)(X / B0) * ((i64)nby * nbz * (B0 * B1 * B2)) + (i64)(X % B0) * (B1 * B2);
+  }
+  ST_HD int ioff(int Y, int Z) const {
+    if constexpr (B0 == 0) return Y * s2 + Z;
+    else return ((Y / B1) * nbz + Z / B2) * (B0 * B1 * B2) + (Y % B1) * B2 + (Z % B2);
+  }
+  ST_HD int bytes(int X, int es) const {
+    if constexpr (B0 == 0) return (int)(plane * es);
+    else return (int)(((i64)nby * nbz * (B0 * B1 * B2) - (i64)(X % B0) * (B1 * B2)) * es);
+  }
 };
 
 // Bijective remap of dispatch order -> logical tile so that XCD x (which
@@ -241,6 +266,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   constexpr int NTZ = T::NTZ, NTY = T::NTY, KY = T::KY, QN = T::QN, NVB = T::max_nvb();
   constexpr int ES = (int)sizeof(S);
   static_assert(P::NT % NTZ == 0, "NT must be a multiple of TZ/VZ");
+  static_assert(P::BR0 == 0 || P::BR2 % VZ == 0, "a vector must not cross a brick");
 
   __shared__ C lds[T::lds_elems()];
 
@@ -250,8 +276,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   const int xb = chunk * a.xch;
   const int xe = (xb + a.xch < a.n0) ? xb + a.xch : a.n0;
   const int s1 = (int)a.s1, s2 = (int)a.s2;
-  const i64 plane = a.s1 * a.s2;
-  const int pbytes = (int)(plane * ES);
+  const Layout<P::BR0, P::BR1, P::BR2> lay{a.s1 * a.s2, s2, a.nby, a.nbz};
   const bool w0 = a.wrap & 1, w1 = a.wrap & 2, w2 = a.wrap & 4;
   const int zc = z0 + tz * VZ;                  // first interior z of my vector
   const bool zin = zc < a.n2;                   // my cells are interior (stored)
@@ -265,7 +290,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
 
   auto in_rsrc = [&](int f, int x) -> Rsrc {
     const int px = w0 ? wrapi(x, a.n0) : x;
-    return make_rsrc(reinterpret_cast<const S*>(a.in[f]) + (i64)(a.h0 + px) * plane, pbytes);
+    return make_rsrc(reinterpret_cast<const S*>(a.in[f]) + lay.base(a.h0 + px), lay.bytes(a.h0 + px, ES));
   };
 
   // my cells of plane x of field f (cells outside the interior but inside
@@ -279,7 +304,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       const int yq = w1 ? wrapi(y, a.n1) : y;
       const bool ok = live && zload && a.h1 + yq < s1 && ty + k * NTY < TY;
       // a queue plane is read once per chunk: optionally non-temporal (P::NTL)
-      const Vec<S, VZ> w = bload<S, VZ, P::NTL ? BUF_NT : 0>(r, ok ? ((a.h1 + yq) * s2 + a.h2 + zq) * ES : BUF_OOB);
+      const Vec<S, VZ> w = bload<S, VZ, P::NTL ? BUF_NT : 0>(r, ok ? lay.ioff(a.h1 + yq, a.h2 + zq) * ES : BUF_OOB);
 #pragma unroll
       for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
     }
@@ -303,7 +328,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
         gy += a.h1;
         gz += a.h2;
         ok = ok && gy >= 0 && gy < s1 && gz >= 0 && gz + VZ <= s2;
-        const Vec<S, VZ> w = bload<S, VZ>(r, ok ? (gy * s2 + gz) * ES : BUF_OOB);
+        const Vec<S, VZ> w = bload<S, VZ>(r, ok ? lay.ioff(gy, gz) * ES : BUF_OOB);
 #pragma unroll
         for (int v = 0; v < VZ; ++v) pb[b][j][v] = Cvt<S, C>::ld(w.v[v]);
       }
@@ -393,7 +418,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     Rsrc ro[NOUT];
 #pragma unroll
     for (int m = 0; m < NOUT; ++m)
-      ro[m] = make_rsrc(reinterpret_cast<S*>(a.out[m]) + (i64)(a.h0 + x) * plane, pbytes);
+      ro[m] = make_rsrc(reinterpret_cast<S*>(a.out[m]) + lay.base(a.h0 + x), lay.bytes(a.h0 + x, ES));
 #pragma unroll
     for (int k = 0; k < KY; ++k) {
       // rows past the tile (when NTY does not divide TY) compute on the
@@ -427,7 +452,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
           }, typename mkseq<NO>::type{});
         }
       }
-      const int off = ((a.h1 + y) * s2 + a.h2 + zc) * ES;
+      const int off = lay.ioff(a.h1 + y, a.h2 + zc) * ES;
       // one unconditional vector store per output.  With PARTIAL (rows whose
       // interior is not a multiple of VZ: batched instances with padded
       // rows) the lane holding a row's ragged end first reads the output's

@@ -230,11 +230,13 @@ def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
 
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,
                 nontemporal: bool = True, prefetch: int = 1, partial: bool = False,
-                ntload: bool = False, tblock: "TBlockConfig | None" = None) -> str:
+                ntload: bool = False, tblock: "TBlockConfig | None" = None, brick: tuple = (0, 0, 0)) -> str:
     """Traits struct of one kernel variant.  ``partial``: rows may end in a
     partial vector (batched instances whose interior is not a multiple of
     the vector width).  ``tblock``: traits of a temporal-blocking variant
-    (levels, ghost widths; the queue holds every accessed plane offset)."""
+    (levels, ghost widths; the queue holds every accessed plane offset).
+    ``brick``: (B0, B1, B2) brick of a brick-major variant in the 3-D map
+    ((0, 0, 0): flat arrays)."""
     plan = plan_accesses(meta)
     bufs = plan["bufs"]
     r3 = radius3(meta)
@@ -256,6 +258,7 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int RX = {r3[0]}, RY = {r3[1]}, RZ = {r3[2]};",
         f"  static constexpr int NTS = {1 if nontemporal else 0}, PD = {int(prefetch)}, "
         f"PARTIAL = {1 if partial else 0}, NTL = {1 if ntload else 0};",
+        f"  static constexpr int BR0 = {int(brick[0])}, BR1 = {int(brick[1])}, BR2 = {int(brick[2])};",
     ]
     if tblock is not None:
         lines.append(f"  static constexpr int TT = {tblock.T}, GY = {tblock.gy}, GZ = {tblock.gz};")
@@ -293,9 +296,14 @@ def tblock_symbol(meta: StencilMeta, cfg: TBlockConfig) -> str:
     return f"{meta.name}_tblock_{cfg.tag}"
 
 
+def brick_symbol(meta: StencilMeta, tag: str, brick: tuple) -> str:
+    return f"{meta.name}_bricks{'x'.join(str(int(b)) for b in brick)}_{tag}"
+
+
 def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage: str,
                     compute: str, temporal=None, nontemporal: bool = True, prefetch: int = 1,
-                    ntload: bool = False, tblocks: Sequence[TBlockConfig] = ()) -> str:
+                    ntload: bool = False, tblocks: Sequence[TBlockConfig] = (),
+                    bricked: Sequence[tuple] = ()) -> str:
     """Full HIP translation unit with one sweep kernel per variant (plus the
     temporal-blocking kernels: ``temporal`` = (config, T) of the LDS-ring
     kernel, ``tblocks`` the register/LDS ones)."""
@@ -320,6 +328,14 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
         out.append(
             f'extern "C" __global__ void __launch_bounds__({cfg.nt}) '
             f"{temporal_symbol(meta, cfg.tag, T)}(st::SweepArgs a) {{ st::temporal_entry<P_{tag}, {T}>(a); }}"
+        )
+        out.append("")
+    for cfg, brick in bricked:
+        tag = f"br_{cfg.tag}"
+        out.append(emit_traits(meta, cfg, storage, compute, tag, nontemporal, prefetch, ntload=ntload, brick=brick))
+        out.append(
+            f'extern "C" __global__ void __launch_bounds__({cfg.nt}) '
+            f"{brick_symbol(meta, cfg.tag, brick)}(st::SweepArgs a) {{ st::sweep_entry<P_{tag}>(a); }}"
         )
         out.append("")
     for tb in tblocks:

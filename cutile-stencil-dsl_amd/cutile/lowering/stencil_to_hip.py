@@ -26,8 +26,8 @@ from typing import Optional, Sequence
 
 from cutile.frontend.ir import find_func
 from cutile.lowering.emitter import CodeEmitter
-from cutile.lowering.hip_emitter import (ELEM_BYTES, KernelConfig, emit_hip_source, kernel_symbol, lds_bytes,
-                                        plan_accesses, radius3, tblock_config, tblock_queue_range,
+from cutile.lowering.hip_emitter import (ELEM_BYTES, KernelConfig, brick_symbol, emit_hip_source, kernel_symbol,
+                                        lds_bytes, plan_accesses, radius3, tblock_config, tblock_queue_range,
                                         tblock_symbol, temporal_symbol, temporal_tile)
 from cutile.lowering.meta import StencilMeta, apply_update, extract_fused_meta, extract_meta
 
@@ -110,9 +110,16 @@ def _attrs(kernel_hints: Optional[dict]) -> str:
     return f"__attribute__((amdgpu_waves_per_eu({int(occ)}))) " if occ else ""
 
 
+def brick_dims(ndim: int, brick_size: int) -> tuple:
+    """Brick extents in the 3-D (stream, row, column) map of an ``ndim``-D array."""
+    b = int(brick_size)
+    return {1: (1, 1, b), 2: (b, 1, b), 3: (b, b, b)}[ndim]
+
+
 def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Sequence[int],
                 temporal_steps: int = 1, boundary_spec=None, kernel_hints: Optional[dict] = None,
-                storage: Optional[str] = None, fused: bool = False, temporal_mode: str = "zero") -> str:
+                storage: Optional[str] = None, fused: bool = False, temporal_mode: str = "zero",
+                brick_size: Optional[int] = None) -> str:
     storage_t, compute_t = storage_compute(meta.dtype, storage)
     vec, scal, xch = make_configs(meta.ndim, tile_sizes, storage_t, (kernel_hints or {}).get("threads"),
                                   (kernel_hints or {}).get("vector"))
@@ -139,10 +146,15 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
         else:
             ty, tz = temporal_tile(meta.ndim, temporal_steps, radius3(meta), compute_t)
             temporal = (KernelConfig(ty=ty, tz=tz, vz=1, nt=256), int(temporal_steps))
+    bricked = []
+    if brick_size:
+        bd = brick_dims(meta.ndim, brick_size)
+        bricked = [(c, bd) for c in dict.fromkeys((vec, scal)) if bd[2] % c.vz == 0]
     src = emit_hip_source(meta, [vec, scal], storage_t, compute_t, temporal,
                           nontemporal=bool((kernel_hints or {}).get("nontemporal", 1)),
                           prefetch=int((kernel_hints or {}).get("prefetch", 1)),
-                          ntload=bool((kernel_hints or {}).get("ntload", 0)), tblocks=tblocks)
+                          ntload=bool((kernel_hints or {}).get("ntload", 0)), tblocks=tblocks,
+                          bricked=bricked)
     attr = _attrs(kernel_hints)
     if attr:
         src = src.replace('extern "C" __global__ void ', f'extern "C" __global__ void {attr}')
@@ -156,6 +168,12 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
     for tb, key in zip(tblocks, ("tblock", "tblock_s")):
         # (symbol, output tile rows, output tile columns, vector, threads)
         variants[key] = (tblock_symbol(meta, tb), tb.oty, tb.otz, tb.vz, tb.nt)
+    for c, bd in bricked:
+        # brick-major sweeps: "brick" at the vector width, "brick_s" scalar
+        variants["brick" if c.vz > 1 or vec.vz == 1 else "brick_s"] = (brick_symbol(meta, c.tag, bd), c.ty, c.tz,
+                                                                      c.vz, c.nt)
+    if bricked and "brick_s" not in variants:
+        variants["brick_s"] = variants["brick"]
     plan = plan_accesses(meta)
     qdepth = max(1, plan["qhi"] - plan["qlo"] + 1)
     nd = meta.ndim
@@ -197,7 +215,7 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
     e.line(f"_KERNEL = StencilKernel({meta.name!r}, {nd}, {meta.num_inputs}, {meta.num_outputs}, "
            f"_HIP_SOURCE, {{k: tuple(v) for k, v in _VARIANTS.items()}}, STORAGE, {int(xch)}, "
            f"queue_depth={qdepth}, temporal_steps={int(temporal_steps) if temporal_steps > 1 else 1}, "
-           f"tblock_depth={tblock_depth})")
+           f"tblock_depth={tblock_depth}, brick={tuple(brick_dims(nd, brick_size)) if brick_size else None})")
     e.line(f"_HALO = ({', '.join(hvars)}{tr})")
 
     if boundary_spec is not None and meta.num_outputs == 1 and not fused:
@@ -238,7 +256,7 @@ def _spec_repr(spec) -> str:
 def lower_stencil_to_python(module, domain=None, tile_sizes=None, halo_widths=None,
                             temporal_steps: int = 1, boundary_spec=None, kernel_hints=None,
                             spatial_term_order=None, storage=None, temporal_mode: str = "zero",
-                            update=None, dt: float = 1.0, targets=None) -> str:
+                            update=None, dt: float = 1.0, targets=None, brick_size=None) -> str:
     func = find_func(module)
     if tile_sizes is None:
         from cutile.passes.tiling import TilingPass, unique_access_count
@@ -256,7 +274,7 @@ def lower_stencil_to_python(module, domain=None, tile_sizes=None, halo_widths=No
     if boundary_spec is None and meta.boundary is not None:
         boundary_spec = meta.boundary
     return emit_module(meta, tile_sizes, halo_widths, temporal_steps, boundary_spec, kernel_hints, storage,
-                       temporal_mode=temporal_mode)
+                       temporal_mode=temporal_mode, brick_size=brick_size)
 
 
 def lower_fused_stencils_to_python(modules: Sequence, tile_sizes=None, halo_widths=None,

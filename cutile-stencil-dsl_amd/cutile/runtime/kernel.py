@@ -53,7 +53,7 @@ class SweepArgs(ctypes.Structure):
         ("n0", ctypes.c_int), ("n1", ctypes.c_int), ("n2", ctypes.c_int),
         ("ntz", ctypes.c_int), ("nty", ctypes.c_int), ("nch", ctypes.c_int),
         ("xch", ctypes.c_int), ("nxcd", ctypes.c_int), ("tmode", ctypes.c_int),
-        ("yslab", ctypes.c_int), ("wrap", ctypes.c_int),
+        ("yslab", ctypes.c_int), ("wrap", ctypes.c_int), ("nby", ctypes.c_int), ("nbz", ctypes.c_int),
     ]
 
 
@@ -227,7 +227,7 @@ class StencilKernel:
 
     def __init__(self, name: str, ndim: int, num_inputs: int, num_outputs: int, source: str,
                  variants: dict, storage: str, xch: int, queue_depth: int = 3, temporal_steps: int = 1,
-                 tblock_depth: int = 0):
+                 tblock_depth: int = 0, brick: Optional[tuple] = None):
         # variants: {"vec": (symbol, ty, tz, vz, nt), "scalar": (...)}; temporal blocking:
         # "tblock" / "tblock_s" (register/LDS kernel, vector / scalar; ty, tz = output tile)
         # or "temporal" (LDS-ring kernel)
@@ -242,6 +242,7 @@ class StencilKernel:
         self.queue_depth = queue_depth
         self.temporal_steps = temporal_steps
         self.tblock_depth = tblock_depth
+        self.brick = tuple(brick) if brick else None     # brick extents in the 3-D map
         self._tb_chunks: dict = {}
         self._launches: dict = {}
         self._modules: dict = {}
@@ -388,6 +389,52 @@ class StencilKernel:
         if stream is None:
             stream = default_stream()
         rt.launch(fn, (nblocks, 1, 1), (self.variants[key][4], 1, 1), args, stream)
+
+    def launch_bricks(self, inputs: Sequence, outputs: Sequence, shape: Sequence[int], halo: Sequence[int],
+                      stream=None, xch: Optional[int] = None) -> None:
+        """One sweep on brick-major arrays (``cutile.runtime.bricks.to_bricks``
+        of flat arrays of ``shape``, halo included): the kernel reads and
+        writes the bricks in place, addressing cell (x, y, z) through its
+        brick; only interior cells of the outputs are written."""
+        torch = _torch()
+        if self.brick is None:
+            raise ValueError(f"launch_{self.name}: compiled without a brick layout (layout='bricked')")
+        shape = tuple(int(x) for x in shape)
+        if len(shape) != self.ndim:
+            raise ValueError(f"launch_{self.name}: expected a {self.ndim}-D shape, got {shape}")
+        b = self.brick[2]
+        want = tuple(-(-x // b) for x in shape) + (b,) * self.ndim
+        dt = getattr(torch, STORAGE_TORCH[self.storage])
+        if len(inputs) != self.num_inputs or len(outputs) != self.num_outputs:
+            raise ValueError(f"launch_{self.name}: expected {self.num_inputs} inputs and {self.num_outputs} outputs")
+        for t in list(inputs) + list(outputs):
+            if not isinstance(t, torch.Tensor) or not t.is_cuda or tuple(t.shape) != want or t.dtype != dt \
+                    or not t.is_contiguous():
+                raise ValueError(f"launch_{self.name}: brick arrays must be contiguous {dt} device tensors of "
+                                 f"shape {want} (bricks of {b} for a {shape} array)")
+        esz = inputs[0].element_size()
+
+        class _View:
+            def __init__(self, t):
+                self.shape, self._p = shape, t.data_ptr()
+
+            def data_ptr(self):
+                return self._p
+
+            def element_size(self):
+                return esz
+
+        vi, vo = [_View(t) for t in inputs], [_View(t) for t in outputs]
+        key = "brick" if self.vector_ok(vi, vo, halo, self.variants["brick"][3]) else "brick_s"
+        args, nblocks = self.args(vi, vo, halo, key, xch)
+        if args is None:
+            return
+        s3 = map3_shape(self.ndim, shape)
+        args.nby = -(-s3[1] // self.brick[1])
+        args.nbz = -(-s3[2] // self.brick[2])
+        fn = self._function(key)
+        rt.launch(fn, (nblocks, 1, 1), (self.variants[key][4], 1, 1), args,
+                  stream if stream is not None else default_stream())
 
     def wrap_mask(self, periodic_axes: Sequence[int], args, need_halo: bool = True) -> int:
         """Bits of the 3-D map for the periodic array axes (the sweep wraps

@@ -359,7 +359,8 @@ def compile(stencil_fn, domain=None, hw=None, pipeline: Optional[Pipeline] = Non
     code = lower_stencil_to_python(ir.clone(), domain=domain, tile_sizes=tiles, halo_widths=halo,
                                    temporal_steps=T, boundary_spec=boundary_spec,
                                    kernel_hints=hints, storage=storage, temporal_mode=temporal_mode,
-                                   update=update, dt=dt, targets=targets)
+                                   update=update, dt=dt, targets=targets,
+                                   brick_size=brick_size if layout == "bricked" else None)
     if layout == "bricked":
         from cutile.lowering.bricked import bricked_suffix
 

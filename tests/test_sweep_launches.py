@@ -75,3 +75,36 @@ def test_suite_sweeps_in_bounds(name):
     r = compile(fn, temporal_blocking=False)
     for domain in suite.DOMAINS[name]:
         _check(r, _single(fn), tuple(d + 2 * h for d, h in zip(domain, r.halo_widths)))
+
+
+@pytest.mark.parametrize("which,b,n", [("diffusion7", 8, (18, 19, 21)), ("star25", 8, (20, 24, 36)),
+                                       ("heat2d", 32, (100, 130)), ("diffusion7", 32, (64, 64, 64))])
+def test_brick_sweeps_in_bounds(which, b, n):
+    """Brick-major launches use the flat variants' tiles on the logical
+    (flat) shape; the brick address map is checked separately
+    (``test_brick_layout.py``)."""
+    from cutile import workloads as w
+
+    fn = getattr(w, which)()
+    r = compile(fn, temporal_blocking=False, layout="bricked", brick_size=b)
+    kern = r.load_module()._KERNEL
+    shape = tuple(a + 2 * h for a, h in zip(n, r.halo_widths))
+
+    class V:
+        def __init__(self):
+            self.shape = shape
+
+        def data_ptr(self):
+            return 256
+
+        def element_size(self):
+            return 8
+
+    key = "brick" if kern.vector_ok([V()], [V()], r.halo_widths, kern.variants["brick"][3]) else "brick_s"
+    a, nblocks = kern.args([V()], [V()], r.halo_widths, key)
+    _, ty, tz, vz, nt = kern.variants[key]
+    assert kern.brick[2] % vz == 0
+    meta = _single(fn)
+    errs = km.check_launch(KernelConfig(ty, tz, vz, nt), plan_accesses(meta), (a.s0, a.s1, a.s2),
+                           (a.h0, a.h1, a.h2), (a.n0, a.n1, a.n2), a.xch, a.yslab, accesses=access_table(meta))
+    assert not errs, errs[:5]
