@@ -193,6 +193,8 @@ def temporal_depths(interior_dev, n, steps=48):
                 torch.cuda.synchronize()
                 kind = "sweep" if st._block is None else (
                     "registers+LDS" if "tblock" in st._block.variants else "LDS rings")
+                if st.block_reps > 1:
+                    kind += f", {st.block_reps} launches of depth {st.T // st.block_reps} per block"
                 out[key] = {"T": T, "dtype": dt_name, "steps": steps, "halo": list(st.halo), "kernel": kind,
                             "ms_per_step": ms / steps, "gcell_updates_per_s": n ** 3 * steps / (ms * 1e-3) / 1e9,
                             "alg_bytes_per_update": 2 * u.element_size() / T, "state_equals_T1": same}

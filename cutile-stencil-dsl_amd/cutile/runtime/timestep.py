@@ -81,13 +81,31 @@ class TimeStepper:
 
         kind = _bc_kind(boundary, find_func(stencil_fn._ir).ndim)
         self.mode = "recompute" if kind == "periodic" else "zero"
-        if self.T > 1 and kind == "other":
-            self.T = 1  # no exact multi-level semantics for these boundaries
+        if self.T > 1 and kind in ("other", "dirichlet"):
+            # no exact multi-level semantics: intermediate levels of a
+            # temporal block hold 0 (or recomputed values) outside the
+            # interior, not a refilled boundary value
+            self.T = 1
         self.single = _compile(stencil_fn, temporal_blocking=False, storage=storage, tile_sizes=tile_sizes,
                                kernel_hints=kernel_hints)
+        self.block_reps = 1
         if self.T > 1:
             self.block = _compile(stencil_fn, temporal_steps=self.T, temporal_mode=self.mode, storage=storage,
                                   kernel_hints=block_hints)
+            if "tblock" not in self.block.load_module()._VARIANTS:
+                # the ghost ring of T levels fits neither registers nor LDS:
+                # a block of T steps runs as T/d launches of the largest depth
+                # d dividing T that fits (same states for periodic and zero
+                # Dirichlet problems: the launches chain through zero or
+                # periodic halos exactly as T single steps do)
+                for d in range(self.T - 1, 1, -1):
+                    if self.T % d:
+                        continue
+                    alt = _compile(stencil_fn, temporal_steps=d, temporal_mode=self.mode, storage=storage,
+                                   kernel_hints=block_hints)
+                    if "tblock" in alt.load_module()._VARIANTS:
+                        self.block, self.block_reps = alt, self.T // d
+                        break
             self.halo = self.block.halo_widths
         else:
             self.block = None
@@ -118,12 +136,14 @@ class TimeStepper:
     def _blk(self, a, b, stream, wrap=()):
         # the halo of `a` is current (every step or block ends with a fill),
         # or `wrap` names the periodic axes the block reads around
-        self._block.launch_temporal([a], [b], self.halo, stream,
-                                    mode=1 if (self.mode == "recompute" and not wrap) else 0,
-                                    validate=False, periodic_axes=wrap)
-        if not wrap and self.policy.mode != "static":   # a Dirichlet halo is never written: it stays valid
-            self.fill(b, stream)
-        return b, a
+        for _ in range(self.block_reps):
+            self._block.launch_temporal([a], [b], self.halo, stream,
+                                        mode=1 if (self.mode == "recompute" and not wrap) else 0,
+                                        validate=False, periodic_axes=wrap)
+            if not wrap and self.policy.mode != "static":   # a Dirichlet halo is never written: it stays valid
+                self.fill(b, stream)
+            a, b = b, a
+        return a, b
 
     def _block_wrap(self) -> tuple:
         """Periodic axes a temporal block reads around (no halo fills between

@@ -125,3 +125,38 @@ def test_graph_keeps_its_buffers_alive():
     ts._graph.launch()
     torch.cuda.synchronize()
     assert all(bool((t == 7.0).all()) for t in held)
+
+
+def test_nonzero_dirichlet_with_a_temporal_depth_matches_march(oracle):
+    """A boundary value other than 0 cannot be carried through the zeroed
+    ghost levels of a temporal block: the stepper steps one at a time and
+    matches the per-step refill definition."""
+    from cutile import workloads as w
+    from cutile.runtime.timestep import TimeStepper
+
+    fn = w.heat2d()
+    spec = w.dirichlet_spec(2, value=1.5)
+    ts = TimeStepper(fn, spec, temporal_steps=4)
+    u0 = w.heat2d_init(64, ts.halo, seed=3)
+    oracle.apply_boundary(u0, spec, ts.halo)
+    got = to_host(ts.march(to_dev(u0), 7))
+    ref = oracle.time_march(u0, fn._fn, 2, ts.halo, 7, spec)
+    assert np.array_equal(got, ref)
+
+
+def test_split_deep_block_matches_march(oracle):
+    """Depth 4 of the radius-4 star runs as two depth-2 launches per block
+    (periodic, wrapped): equal to 8 single periodic steps."""
+    from cutile import workloads as w
+    from cutile.runtime.timestep import TimeStepper
+
+    fn = w.star25()
+    spec = w.periodic_spec(3)
+    ts = TimeStepper(fn, spec, temporal_steps=4)
+    assert ts.block_reps == 2
+    print(f"star25 depth 4 as 2 x 2, halo {ts.halo}", flush=True)
+    u0 = w.sinusoid_init((20, 24, 40), ts.halo, seed=4)
+    oracle.apply_boundary(u0, spec, ts.halo)
+    got = to_host(ts.march(to_dev(u0), 8))
+    ref = oracle.time_march(u0, fn._fn, 3, ts.halo, 8, spec)
+    assert np.array_equal(got, ref)
