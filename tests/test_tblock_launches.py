@@ -71,6 +71,7 @@ DEVICE_CASES = [
     ("heat2d", 3, "validate", 0),       # CompileResult.validate: 64 + 2*T*h per axis, halo included
     # TimeStepper on a periodic problem: blocks read around every axis
     ("star25", 2, (24, 20, 36), "wrap"),
+    ("star25", 2, (20, 24, 40), "wrap"),          # a depth-4 block split into two depth-2 launches
     ("diffusion7", 4, (30, 18, 50), "wrap"),
     ("diffusion7", 2, (16, 20, 24), "wrap"),
 ]
