@@ -85,7 +85,7 @@ def gvec_rc(b, i):
     return r, (k if k < sv else w - 2 * sv + k)
 
 
-def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, edges=False):
+def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, edges=False, rad=None):
     """Run one launch of the temporal kernel on ``u`` (the padded input).
 
     ``cfg``: the variant's TBlockConfig; ``plan``: ``plan_accesses`` of the
@@ -94,6 +94,9 @@ def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, ed
     (bounds of a large launch; the output is then partial).  Returns
     (output array, list of error strings)."""
     TT, TY, TZ, VZ, NT = cfg.T, cfg.ty, cfg.tz, cfg.vz, cfg.nt
+    if rad is None:      # stencil radius in the 3-D map, from the LDS buffers and queue range
+        rad = (max(-qrange[0], qrange[1]), max([b[2] for b in plan["bufs"]] + [0]),
+               max([b[3] for b in plan["bufs"]] + [0]))
     GY, GZ, OTY, OTZ = cfg.gy, cfg.gz, cfg.oty, cfg.otz
     QLO, QHI = qrange
     QN, D = QHI - QLO + 1, QHI + 1
@@ -234,6 +237,12 @@ def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, ed
 
                 val = np.broadcast_to(np.asarray(f(_Proxy(ndim, get), *([0] * ndim)), dtype=np.float64),
                                       (TY, TZ))
+                # the kernel may skip (and zero) cells outside the output tile
+                # widened by (TT-L) radii; zero them all here, so a needed
+                # cell skipped there fails the comparison with the oracle
+                wy, wz = (TT - L) * rad[1], (TT - L) * rad[2]
+                need = (ly >= GY - wy) & (ly < GY + OTY + wy) & (lzv + VZ > GZ - wz) & (lzv < GZ + OTZ + wz)
+                val = np.where(need, val, 0.0)
                 if L == TT:
                     if p >= xb:
                         if not (0 <= p < n0):
