@@ -75,7 +75,7 @@ class BatchedStencil:
     """One persistent kernel for many arrays of the same stencil."""
 
     def __init__(self, stencil_fn, storage: Optional[str] = None, buckets: Sequence[Bucket] = None,
-                 occupancy: int = 8):
+                 occupancy: int = 8, threads: int = 256):
         from cutile.frontend.ir import find_func
         from cutile.lowering.hip_emitter import KernelConfig, emit_traits
         from cutile.lowering.meta import extract_fused_meta, extract_meta
@@ -105,6 +105,7 @@ class BatchedStencil:
                 buckets = [Bucket("rows", 4, 64 * self.vz, 16, None)]
         self.buckets = list(buckets)
         self.occupancy = occupancy
+        self.threads = int(threads)
         self.num_inputs = len(self.inputs)
         self.num_outputs = len(self.outputs)
         parts = ['#include "stencil_core.h"', ""]
@@ -115,9 +116,9 @@ class BatchedStencil:
             meta = (extract_fused_meta([f._ir for f in fns], mapping=mp) if len(fns) > 1
                     else extract_meta(func, mapping=mp))
             ntz = b.tz // self.vz
-            if 256 % ntz:
-                raise ValueError(f"bucket {b}: {ntz} threads per row must divide 256")
-            cfg = KernelConfig(ty=b.ty, tz=b.tz, vz=self.vz, nt=256)
+            if threads % ntz:
+                raise ValueError(f"bucket {b}: {ntz} threads per row must divide {threads}")
+            cfg = KernelConfig(ty=b.ty, tz=b.tz, vz=self.vz, nt=threads)
             self.cfgs.append(cfg)
             parts.append(emit_traits(meta, cfg, self.storage, self.compute, f"b{k}", partial=True))
             cases.append(f"      case {k}: st::sweep_tile<P_b{k}>(a, tz, ty, ch); break;")
@@ -128,7 +129,7 @@ class BatchedStencil:
         # search over first[] (uniform, so scalar loads).  The host uploads
         # only per-instance records, so the list is cheap to rebuild.
         parts.append(
-            f'extern "C" __global__ void __launch_bounds__(256) {self.symbol}('
+            f'extern "C" __global__ void __launch_bounds__({threads}) {self.symbol}('
             "const st::SweepArgs* __restrict__ insts, const int* __restrict__ first, "
             "const int* __restrict__ bucket, int ninst, int nitems) {\n"
             "  for (int i = blockIdx.x; i < nitems; i += gridDim.x) {\n"
@@ -270,7 +271,7 @@ class BatchedStencil:
 
         args = _Args(plan.table.data_ptr(), plan.first.data_ptr(), plan.bucket.data_ptr(), plan.instances,
                      plan.nitems)
-        rt.launch(self._function(), (plan.grid, 1, 1), (256, 1, 1), args,
+        rt.launch(self._function(), (plan.grid, 1, 1), (self.threads, 1, 1), args,
                   stream if stream is not None else default_stream())
 
 
