@@ -168,6 +168,14 @@ template <class P> struct Tab {
 // Per-cell view handed to the generated expression.  The register queue is
 // a ring: at rotation R, slot (i + R) % QN holds plane x + QLO + i.  KD is
 // the number of cells per thread (the queue's third extent).
+// bytes of LDS the largest of several sweep instantiations needs
+template <class... P> constexpr int sweep_lds_bytes() {
+  const int bytes[] = {16, (int)(Tab<P>::lds_elems() * sizeof(typename P::C))...};
+  int m = 0;
+  for (int b : bytes) m = b > m ? b : m;
+  return m;
+}
+
 template <class P, int R, int KD = Tab<P>::KY> struct Cell {
   using C = typename P::C;
   using T = Tab<P>;
@@ -256,8 +264,10 @@ ST_HD int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 // along it wrap around instead of reading the halo, so no halo fill is
 // needed between steps (the output's halo is not written).  The host allows
 // it when n_d >= h_d and, for the vector variant, n2 % VZ == 0.
+// lds: the workgroup's LDS, at least Tab<P>::lds_elems() elements (kernels
+// that dispatch to several instantiations share one pool sized to the largest)
 template <class P>
-ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
+ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typename P::C* lds) {
   using S = typename P::S;
   using C = typename P::C;
   using T = Tab<P>;
@@ -267,8 +277,6 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   constexpr int ES = (int)sizeof(S);
   static_assert(P::NT % NTZ == 0, "NT must be a multiple of TZ/VZ");
   static_assert(P::BR0 == 0 || P::BR2 % VZ == 0, "a vector must not cross a brick");
-
-  __shared__ C lds[T::lds_elems()];
 
   const int tid = threadIdx.x;
   const int tz = tid % NTZ, ty = tid / NTZ;
@@ -890,14 +898,15 @@ ST_HD void temporal_entry(const SweepArgs& a) {
 
 template <class P>
 ST_HD void sweep_entry(const SweepArgs& a) {
+  __shared__ typename P::C lds[Tab<P>::lds_elems()];
   const int n = a.ntz * a.nty * a.nch;
   const int t = xcd_swizzle((int)blockIdx.x, n, a.nxcd);
   const int tile_z = t % a.ntz;
   const int rest = t / a.ntz;
   if (a.yslab) {
-    sweep_tile<P>(a, tile_z, rest / a.nch, rest % a.nch);
+    sweep_tile<P>(a, tile_z, rest / a.nch, rest % a.nch, lds);
   } else {
-    sweep_tile<P>(a, tile_z, rest % a.nty, rest / a.nty);
+    sweep_tile<P>(a, tile_z, rest % a.nty, rest / a.nty, lds);
   }
 }
 

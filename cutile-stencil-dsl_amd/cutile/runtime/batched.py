@@ -121,7 +121,7 @@ class BatchedStencil:
             cfg = KernelConfig(ty=b.ty, tz=b.tz, vz=self.vz, nt=threads)
             self.cfgs.append(cfg)
             parts.append(emit_traits(meta, cfg, self.storage, self.compute, f"b{k}", partial=True))
-            cases.append(f"      case {k}: st::sweep_tile<P_b{k}>(a, tz, ty, ch); break;")
+            cases.append(f"      case {k}: st::sweep_tile<P_b{k}>(a, tz, ty, ch, reinterpret_cast<P_b{k}::C*>(pool)); break;")
         self.symbol = f"{self.name}_{self.storage}_batched"
         # The work-list is implicit: instance k owns items first[k] ..
         # first[k+1]-1, decoded into (tile z, tile y, chunk) from its own
@@ -132,6 +132,11 @@ class BatchedStencil:
             f'extern "C" __global__ void __launch_bounds__({threads}) {self.symbol}('
             "const st::SweepArgs* __restrict__ insts, const int* __restrict__ first, "
             "const int* __restrict__ bucket, int ninst, int nitems) {\n"
+            # one LDS pool for every bucket, sized to the largest: a workgroup
+            # runs one item at a time, and each item's first LDS write
+            # follows a workgroup barrier
+            f"  __shared__ __attribute__((aligned(16))) unsigned char pool[st::sweep_lds_bytes<"
+            f"{', '.join(f'P_b{k}' for k in range(len(self.buckets)))}>()];\n"
             "  for (int i = blockIdx.x; i < nitems; i += gridDim.x) {\n"
             "    int lo = 0, hi = ninst - 1;\n"
             "    while (lo < hi) {\n"
