@@ -59,6 +59,12 @@ def upload(a: np.ndarray):
     return host.to("cuda", non_blocking=True), host
 
 
+class _BatchArgs(ctypes.Structure):
+    """Kernel arguments of a batched launch (the kernel's parameter list)."""
+    _fields_ = [("insts", ctypes.c_void_p), ("first", ctypes.c_void_p), ("bucket", ctypes.c_void_p),
+                ("ninst", ctypes.c_int), ("nitems", ctypes.c_int)]
+
+
 @dataclass
 class BatchPlan:
     table: object                 # device uint8 tensor holding SweepArgs[]
@@ -269,12 +275,7 @@ class BatchedStencil:
 
         if plan.nitems == 0:
             return
-
-        class _Args(ctypes.Structure):
-            _fields_ = [("insts", ctypes.c_void_p), ("first", ctypes.c_void_p), ("bucket", ctypes.c_void_p),
-                        ("ninst", ctypes.c_int), ("nitems", ctypes.c_int)]
-
-        args = _Args(plan.table.data_ptr(), plan.first.data_ptr(), plan.bucket.data_ptr(), plan.instances,
+        args = _BatchArgs(plan.table.data_ptr(), plan.first.data_ptr(), plan.bucket.data_ptr(), plan.instances,
                      plan.nitems)
         rt.launch(self._function(), (plan.grid, 1, 1), (self.threads, 1, 1), args,
                   stream if stream is not None else default_stream())

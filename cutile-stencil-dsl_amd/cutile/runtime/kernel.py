@@ -160,7 +160,8 @@ def cu_count() -> int:
 
 
 def chunk_length(n0: int, tiles: int, default: int, queue_depth: int, per_cu: int = 1) -> int:
-    """Planes per workgroup along the streaming axis.
+    """Planes per workgroup along the streaming axis (1-D and 2-D sweeps;
+    3-D sweeps use ``sweep_chunk_length``).
 
     The grid gets about ``per_cu`` workgroups per CU: ``ceil(per_cu * CUs /
     tiles)`` chunks of ``ceil(n0 / chunks)`` planes, capped at the compiled
@@ -181,6 +182,27 @@ def chunk_length(n0: int, tiles: int, default: int, queue_depth: int, per_cu: in
     if x < floor and max(1, tiles) * -(-n0 // floor) >= slots:
         x = floor
     return max(1, min(x, n0))
+
+
+def sweep_chunk_length(n0: int, tiles: int, default: int, queue_depth: int) -> int:
+    """Planes per workgroup of a 3-D sweep: one long-streaming workgroup
+    per CU, the chunk count chosen by the round model of
+    ``tblock_chunk_length`` (prologue of ``queue_depth - 1`` planes per
+    chunk, one slot per CU).  A grid a little larger than the CU count
+    leaves a few CUs with two workgroups and the launch takes about twice as
+    long: at 384^3 the 7-point sweep's 288 workgroups (48 tiles x 6 chunks)
+    ran at 406 Gcell/s against 597 at 256^3, where 256 workgroups fill the
+    CUs once.  Of chunk counts within 3 % of the best modelled time the
+    smallest is taken (fewer prologue re-reads)."""
+    slots, pro = cu_count(), max(0, queue_depth - 1)
+    cands = []
+    for c in range(max(1, -(-n0 // max(1, default))), n0 + 1):
+        x = -(-n0 // c)
+        cands.append((-(-(tiles * c) // slots) * (x + pro), c, x))
+        if tiles * c > 64 * slots:
+            break
+    tmin = min(t for t, _, _ in cands)
+    return max(1, min((c, x) for t, c, x in cands if t <= 1.03 * tmin)[1])
 
 
 def tblock_chunk_length(n0: int, tiles: int, default: int, depth: int, slots: int) -> int:
@@ -356,7 +378,12 @@ class StencilKernel:
         else:
             # the LDS-ring temporal kernel warms up (queue_depth - 1) * T planes per chunk
             depth = self.queue_depth if key != "temporal" else (self.queue_depth - 1) * self.temporal_steps + 1
-            x = xch if xch else chunk_length(n[0], a.ntz * a.nty, self.xch, depth, 2 if self.ndim == 2 else 1)
+            if xch:
+                x = xch
+            elif self.ndim == 3:
+                x = sweep_chunk_length(n[0], a.ntz * a.nty, self.xch, depth)
+            else:
+                x = chunk_length(n[0], a.ntz * a.nty, self.xch, depth, 2)
         a.nch = -(-n[0] // x)
         a.xch = x
         a.nxcd = xcd_count(current_arch())

@@ -108,3 +108,13 @@ def test_brick_sweeps_in_bounds(which, b, n):
     errs = km.check_launch(KernelConfig(ty, tz, vz, nt), plan_accesses(meta), (a.s0, a.s1, a.s2),
                            (a.h0, a.h1, a.h2), (a.n0, a.n1, a.n2), a.xch, a.yslab, accesses=access_table(meta))
     assert not errs, errs[:5]
+
+
+@pytest.mark.parametrize("n0,tiles,qd,wgs", [(512, 128, 9, 256), (512, 64, 3, 256), (384, 48, 3, 240),
+                                            (320, 40, 3, 240), (448, 56, 3, 504), (256, 16, 3, 256)])
+def test_3d_sweep_chunks_fill_whole_rounds(n0, tiles, qd, wgs):
+    """3-D sweeps launch whole rounds of one workgroup per CU: the 512^3
+    benchmark grids stay at 256 workgroups, and 384^3 takes 240 workgroups
+    of 77 planes instead of 288 of 64 (a second, nearly empty round)."""
+    x = K.sweep_chunk_length(n0, tiles, 256, qd)
+    assert tiles * -(-n0 // x) == wgs
