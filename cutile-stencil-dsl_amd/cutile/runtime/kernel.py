@@ -493,8 +493,8 @@ class StencilKernel:
         # repeated launches on the same arrays (a time loop) reuse the
         # validated, packed arguments: the signature holds everything the
         # checks and the arguments depend on
-        sig = (tuple((t.data_ptr(), tuple(t.shape), t.stride(), t.dtype, t.get_device())
-                     for t in (*inputs, *outputs)), tuple(halo), xch, tuple(periodic_axes))
+        sig = (tuple([(t.data_ptr(), t.shape, t.stride(), t.dtype, t.get_device()) for t in (*inputs, *outputs)]),
+               tuple(halo), xch, tuple(periodic_axes))
         hit = self._launches.get(sig)
         if hit is None:
             if validate:
@@ -505,11 +505,13 @@ class StencilKernel:
                 return
             if periodic_axes:
                 args.wrap = self.wrap_mask(periodic_axes, args)
-            hit = (self._function(key), nblocks, self.variants[key][4], args)
+            # everything st_launch takes but the stream (args stays referenced)
+            hit = (self._function(key), nblocks, self.variants[key][4], ctypes.addressof(args), ctypes.sizeof(args),
+                   args)
             if len(self._launches) >= 256:
                 self._launches.clear()
             self._launches[sig] = hit
-        fn, nblocks, nt, args = hit
+        fn, nblocks, nt, addr, size, _args = hit
         if stream is None:
             stream = default_stream()
-        rt.launch(fn, (nblocks, 1, 1), (nt, 1, 1), args, stream)
+        rt.check(rt.lib().st_launch(fn, nblocks, 1, 1, nt, 1, 1, 0, stream, addr, size), "st_launch")
