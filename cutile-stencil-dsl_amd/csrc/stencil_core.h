@@ -132,6 +132,19 @@ template <class P> struct Tab {
   static constexpr int rows(int b) { return P::TY + 2 * P::BRY[b]; }
   static constexpr int cols(int b) { return P::TZ + 2 * rzv(b); }
   static constexpr int vpr(int b) { return cols(b) / P::VZ; }
+  // LDS row pitch of buffer b, in elements.  A wave's lanes walk the tile
+  // row by row, NTZ lanes a row.  Where a row ends inside a 32-lane span
+  // (NTZ not a multiple of 32), the row is padded so that the pitch is
+  // congruent to the tile width modulo 256 bytes: the lane after a row wrap
+  // then continues in the bank after the previous lane's, and one LDS access
+  // of a wave stays free of bank conflicts.  Unpadded, the halo columns
+  // shift the next row's lanes onto the banks of the lanes before them.
+  static constexpr int pitch(int b) {
+    if (NTZ % 32 == 0) return cols(b);
+    int p = cols(b);
+    while (((p - P::TZ) * (int)sizeof(typename P::C)) % 256 != 0) ++p;
+    return p;
+  }
   static constexpr int svec(int b) { return rzv(b) / P::VZ; }
   // the tile interior of buffer b is copied from the register queue (only
   // the halo ring is fetched from memory)
@@ -142,8 +155,8 @@ template <class P> struct Tab {
     return fromq(b) ? 2 * P::BRY[b] * vpr(b) + P::TY * 2 * svec(b) : rows(b) * vpr(b);
   }
   static constexpr int nvb(int b) { return (gvecs(b) + P::NT - 1) / P::NT; }
-  static constexpr int off(int b) { int o = 0; for (int k = 0; k < b; ++k) o += rows(k) * cols(k); return o; }
-  static constexpr int lds_elems() { int o = 0; for (int k = 0; k < P::NB; ++k) o += rows(k) * cols(k); return o > 0 ? o : 1; }
+  static constexpr int off(int b) { int o = 0; for (int k = 0; k < b; ++k) o += rows(k) * pitch(k); return o; }
+  static constexpr int lds_elems() { int o = 0; for (int k = 0; k < P::NB; ++k) o += rows(k) * pitch(k); return o > 0 ? o : 1; }
   static constexpr int max_nvb() { int m = 1; for (int k = 0; k < P::NB; ++k) m = nvb(k) > m ? nvb(k) : m; return m; }
   static constexpr int buf(int f, int dx) {
     for (int k = 0; k < P::NB; ++k) if (P::BF[k] == f && P::BDX[k] == dx) return k;
@@ -193,11 +206,11 @@ template <class P, int R, int KD = Tab<P>::KY> struct Cell {
       // once the per-vector loop is unrolled
       if (v + DZ >= 0 && v + DZ < P::VZ) return q[F][(DX - P::QLO + R) % T::QN][k][v + DZ];
       constexpr int b = T::buf(F, DX);
-      return lds[T::off(b) + (ly + P::BRY[b]) * T::cols(b) + (lz + T::rzv(b) + DZ)];
+      return lds[T::off(b) + (ly + P::BRY[b]) * T::pitch(b) + (lz + T::rzv(b) + DZ)];
     } else {
       constexpr int b = T::buf(F, DX);
       static_assert(b >= 0, "access has no LDS buffer");
-      return lds[T::off(b) + (ly + P::BRY[b] + DY) * T::cols(b) + (lz + T::rzv(b) + DZ)];
+      return lds[T::off(b) + (ly + P::BRY[b] + DY) * T::pitch(b) + (lz + T::rzv(b) + DZ)];
     }
   }
 };
@@ -353,7 +366,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
 #pragma unroll
           for (int k = 0; k < KY; ++k) {
             if (ty + k * NTY < TY) {
-              C* d = base + (P::BRY[b] + ty + k * NTY) * T::cols(b) + T::rzv(b) + tz * VZ;
+              C* d = base + (P::BRY[b] + ty + k * NTY) * T::pitch(b) + T::rzv(b) + tz * VZ;
 #pragma unroll
               for (int v = 0; v < VZ; ++v) d[v] = q[P::BF[b]][(P::BDX[b] - P::QLO + R) % QN][k][v];
             }
@@ -366,7 +379,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
         if (j < T::nvb(b) && i < T::gvecs(b)) {
           int r, c;
           T::gvec_rc(b, i, r, c);
-          C* d = base + r * T::cols(b) + c * VZ;
+          C* d = base + r * T::pitch(b) + c * VZ;
 #pragma unroll
           for (int v = 0; v < VZ; ++v) d[v] = pb[b][j][v];
         }
@@ -685,7 +698,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
           if (act[k]) {
-            C* d = base + (P::BRY[b] + ly[k]) * T::cols(b) + T::rzv(b) + lz[k];
+            C* d = base + (P::BRY[b] + ly[k]) * T::pitch(b) + T::rzv(b) + lz[k];
 #pragma unroll
             for (int v = 0; v < VZ; ++v) d[v] = q[L][sl][k][v];
           }
@@ -697,7 +710,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
             if (j < T::nvb(b) && i < T::gvecs(b)) {
               int rr, c;
               T::gvec_rc(b, i, rr, c);
-              C* d = base + rr * T::cols(b) + c * VZ;
+              C* d = base + rr * T::pitch(b) + c * VZ;
 #pragma unroll
               for (int v = 0; v < VZ; ++v) d[v] = pb[b][j][v];
             }

@@ -60,13 +60,26 @@ def plan_accesses(meta: StencilMeta) -> dict:
     return {"qlo": qlo, "qhi": qhi, "qf": qf, "bufs": [(f, dx, ry, rz) for (f, dx), (ry, rz) in blist]}
 
 
+def lds_pitch(tz: int, vz: int, rzv: int, elem_bytes: int) -> int:
+    """LDS row pitch (elements) of a plane buffer, as ``st::Tab::pitch``:
+    the row (tile plus halo columns) is padded to a pitch congruent to the
+    tile width modulo 256 bytes when a row of lanes (tz / vz of them) does
+    not end on a 32-lane boundary."""
+    p = tz + 2 * rzv
+    if (tz // vz) % 32 == 0:
+        return p
+    while ((p - tz) * elem_bytes) % 256:
+        p += 1
+    return p
+
+
 def lds_bytes(meta: StencilMeta, cfg: KernelConfig, compute: str) -> int:
     plan = plan_accesses(meta)
     cb = ELEM_BYTES[compute]
     tot = 0
     for f, dx, ry, rz in plan["bufs"]:
         rzv = -(-rz // cfg.vz) * cfg.vz
-        tot += (cfg.ty + 2 * ry) * (cfg.tz + 2 * rzv) * cb
+        tot += (cfg.ty + 2 * ry) * lds_pitch(cfg.tz, cfg.vz, rzv, cb) * cb
     return tot
 
 

@@ -18,7 +18,17 @@ import itertools
 import numpy as np
 
 
-def _tab(cfg, plan):
+def lds_pitch(tz, vz, rzv, elem_bytes):
+    # row pitch of an LDS plane buffer: padded to tile width mod 256 B when a
+    # row of tz / vz lanes does not end on a 32-lane boundary
+    p = tz + 2 * rzv
+    if (tz // vz) % 32:
+        while ((p - tz) * elem_bytes) % 256:
+            p += 1
+    return p
+
+
+def _tab(cfg, plan, elem_bytes=8):
     NT, TY, TZ, VZ = cfg.nt, cfg.ty, cfg.tz, cfg.vz
     NTZ = TZ // VZ
     NTY = NT // NTZ
@@ -30,13 +40,14 @@ def _tab(cfg, plan):
     for (f, dx, ry, rz) in bufs:
         rzv = -(-rz // VZ) * VZ
         rows, cols = TY + 2 * ry, TZ + 2 * rzv
+        pitch = lds_pitch(TZ, VZ, rzv, elem_bytes)
         fromq = hasq and plan["qf"][f] and plan["qlo"] <= dx <= plan["qhi"]
         w = cols // VZ
         sv = rzv // VZ
         gvecs = 2 * ry * w + TY * 2 * sv if fromq else rows * w
-        t["bufs"].append(dict(f=f, dx=dx, ry=ry, rz=rz, rzv=rzv, rows=rows, cols=cols, w=w, sv=sv,
+        t["bufs"].append(dict(f=f, dx=dx, ry=ry, rz=rz, rzv=rzv, rows=rows, cols=cols, pitch=pitch, w=w, sv=sv,
                               fromq=fromq, gvecs=gvecs, nvb=-(-gvecs // NT), off=off))
-        off += rows * cols
+        off += rows * pitch
     t["lds"] = max(off, 1)
     t["nvb"] = max([1] + [b["nvb"] for b in t["bufs"]])
     return t
@@ -52,12 +63,13 @@ def queue_planes(xb, xe, qlo, qhi, pd):
     return planes
 
 
-def check_launch(cfg, plan, shape3, halo3, n3, xch, yslab, nxcd=8, accesses=None, tiles="edges", pd=1, wrap=0):
+def check_launch(cfg, plan, shape3, halo3, n3, xch, yslab, nxcd=8, accesses=None, tiles="edges", pd=1, wrap=0,
+                 elem_bytes=8):
     """Return a list of out-of-bounds descriptions (empty when clean)."""
     s0, s1, s2 = shape3
     h0, h1, h2 = halo3
     n0, n1, n2 = n3
-    t = _tab(cfg, plan)
+    t = _tab(cfg, plan, elem_bytes)
     NT, TY, TZ, VZ, NTZ, NTY, KY = (t[k] for k in ("NT", "TY", "TZ", "VZ", "NTZ", "NTY", "KY"))
     qlo, qhi = plan["qlo"], plan["qhi"]
     hasq = qhi >= qlo
@@ -106,13 +118,13 @@ def check_launch(cfg, plan, shape3, halo3, n3, xch, yslab, nxcd=8, accesses=None
             ok.append(m)
         rc[bi] = (np.array(rr), np.array(cc), np.array(ok))
         r_, c_, m_ = rc[bi]
-        idx = b["off"] + r_ * b["cols"] + c_ * VZ
+        idx = b["off"] + r_ * b["pitch"] + c_ * VZ
         lcheck("LDS halo store", idx, m_)
         lcheck("LDS halo store (end)", idx + VZ - 1, m_)
         if b["fromq"]:
             for k in range(KY):
                 m = ty + k * NTY < TY
-                d = b["off"] + (b["ry"] + ty + k * NTY) * b["cols"] + b["rzv"] + tz * VZ
+                d = b["off"] + (b["ry"] + ty + k * NTY) * b["pitch"] + b["rzv"] + tz * VZ
                 lcheck("LDS interior store", d, m)
                 lcheck("LDS interior store (end)", d + VZ - 1, m)
 
@@ -154,7 +166,7 @@ def check_launch(cfg, plan, shape3, halo3, n3, xch, yslab, nxcd=8, accesses=None
                     continue
                 b = next(bb for bb in t["bufs"] if bb["f"] == f and bb["dx"] == dx)
                 for v in range(VZ):
-                    idx = b["off"] + (lyc + b["ry"] + dy) * b["cols"] + (tz * VZ + v + b["rzv"] + dz)
+                    idx = b["off"] + (lyc + b["ry"] + dy) * b["pitch"] + (tz * VZ + v + b["rzv"] + dz)
                     lcheck("LDS read", idx, np.ones_like(m))
             for x in (xb, xe - 1):
                 pcheck(f"output store plane {x}", x, (h1 + y) * s2 + h2 + zc, m)

@@ -51,7 +51,8 @@ def test_tuned_tiles_in_bounds(meta, tile, threads, pd, n):
         if not rows_ok:
             continue
         for x in (xch, 64, 16):
-            errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=acc, pd=pd)
+            errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=acc, pd=pd,
+                                   elem_bytes=8 if meta.dtype == "float64" else 4)
             assert not errs, errs[:5]
 
 
@@ -83,7 +84,8 @@ def test_autotune_candidates_in_bounds(meta, tile, threads):
             if shape3[2] % cfg.vz:
                 continue
             for x in (8, 64):
-                errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=acc)
+                errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=acc,
+                                       elem_bytes=8 if meta.dtype == "float64" else 4)
                 assert not errs, errs[:5]
 
 
@@ -117,7 +119,8 @@ def test_random_stencil_launches_in_bounds(seed):
         if s3[2] % cfg.vz or h3[2] % cfg.vz:
             continue
         for x in (xch, 4, 1):
-            errs = km.check_launch(cfg, plan, s3, h3, n3, x, 0, accesses=acc)
+            errs = km.check_launch(cfg, plan, s3, h3, n3, x, 0, accesses=acc,
+                                   elem_bytes=8 if meta.dtype == "float64" else 4)
             assert not errs, errs[:5]
 
 
@@ -139,5 +142,6 @@ def test_wrapped_periodic_reads_in_bounds(n):
             if shape3[2] % cfg.vz:
                 continue
             for x in (xch, 3):
-                errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=access_table(meta), wrap=7)
+                errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=access_table(meta), wrap=7,
+                                       elem_bytes=8 if meta.dtype == "float64" else 4)
                 assert not errs, errs[:5]

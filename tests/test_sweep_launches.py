@@ -34,7 +34,8 @@ def _check(r, meta, shape):
     _, ty, tz, vz, nt = kern.variants[key]
     cfg = KernelConfig(ty, tz, vz, nt)
     errs = km.check_launch(cfg, plan_accesses(meta), (a.s0, a.s1, a.s2), (a.h0, a.h1, a.h2), (a.n0, a.n1, a.n2),
-                           a.xch, a.yslab, accesses=access_table(meta), wrap=a.wrap)
+                           a.xch, a.yslab, accesses=access_table(meta), wrap=a.wrap,
+                           elem_bytes=8 if kern.storage == "float64" else 4)
     assert not errs, errs[:5]
     assert nblocks == a.ntz * a.nty * a.nch
 
@@ -106,7 +107,8 @@ def test_brick_sweeps_in_bounds(which, b, n):
     assert kern.brick[2] % vz == 0
     meta = _single(fn)
     errs = km.check_launch(KernelConfig(ty, tz, vz, nt), plan_accesses(meta), (a.s0, a.s1, a.s2),
-                           (a.h0, a.h1, a.h2), (a.n0, a.n1, a.n2), a.xch, a.yslab, accesses=access_table(meta))
+                           (a.h0, a.h1, a.h2), (a.n0, a.n1, a.n2), a.xch, a.yslab, accesses=access_table(meta),
+                           elem_bytes=8 if kern.storage == "float64" else 4)
     assert not errs, errs[:5]
 
 
