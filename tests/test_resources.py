@@ -68,3 +68,26 @@ def test_random_stencils_build_within_lds(seed):
     res = regcheck.code_object_resources(rt.compile_only(r.hip_source, compile_options("gfx950")))
     for k, v in res.items():
         assert v.lds_bytes <= 160 * 1024, (k, v)
+
+
+@pytest.mark.parametrize("fn,ty,tz,vz,nt", [
+    (lambda: w.star25(), 8, 40, 2, 160),          # 20 lanes a row: padded pitch
+    (lambda: w.star25(), 32, 64, 2, 512),         # 32 lanes a row: unpadded
+    (lambda: w.diffusion7("float32"), 16, 64, 4, 256),   # 16 lanes a row, 4-byte cells
+    (lambda: w.heat2d(), 1, 512, 2, 256),
+])
+def test_sweep_lds_matches_the_host_estimate(fn, ty, tz, vz, nt):
+    """The LDS the compiled sweep reserves equals the emitter's estimate,
+    row pitch padding included (the estimate decides tiles and occupancy)."""
+    from cutile.frontend.ir import find_func
+    from cutile.lowering.hip_emitter import KernelConfig, emit_hip_source, lds_bytes
+    from cutile.lowering.meta import extract_meta
+    from cutile.lowering.stencil_to_hip import storage_compute
+
+    meta = extract_meta(find_func(fn()._ir))
+    storage, compute = storage_compute(meta.dtype, None)
+    cfg = KernelConfig(ty=ty, tz=tz, vz=vz, nt=nt)
+    src = emit_hip_source(meta, [cfg], storage, compute)
+    res = regcheck.code_object_resources(rt.compile_only(src, compile_options("gfx950")))
+    (v,) = res.values()
+    assert v.lds_bytes == lds_bytes(meta, cfg, compute)
