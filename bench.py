@@ -464,13 +464,10 @@ def serving_c5(seconds=10.0, rate=200.0, seed=0):
     Gray-Scott, 80 % float32) for ``seconds`` of arrivals: p50/p99 latency
     and aggregate Gcell-updates/s; every class's kernels are built before
     the clock starts."""
-    from cutile.serving import StencilServer, generate_jobs
+    from cutile.serving import StencilServer, generate_jobs, warmup_jobs
 
     srv = StencilServer()
-    warm = generate_jobs(50.0, 0.2, seed + 1)
-    for j in warm:
-        j.steps = 2
-    srv.run(warm, realtime=False)
+    srv.run(warmup_jobs(), realtime=False)
     jobs = generate_jobs(rate, seconds, seed)
     stats = srv.run(jobs, realtime=True, log=log)
     stats.update(rate=rate, seconds=seconds, offered_jobs=len(jobs),

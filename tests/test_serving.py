@@ -31,7 +31,8 @@ def _mix():
 
     spec = [("heat2d", "float32", 70, 12), ("heat2d", "float64", 200, 25), ("diffusion3d", "float32", 20, 9),
             ("diffusion3d", "float64", 33, 17), ("grayscott", "float32", 50, 30), ("grayscott", "float64", 130, 6),
-            ("heat2d", "float32", 333, 3), ("grayscott", "float32", 129, 21), ("diffusion3d", "float32", 45, 14)]
+            ("heat2d", "float32", 333, 3), ("grayscott", "float32", 129, 21), ("diffusion3d", "float32", 45, 14),
+            ("diffusion3d", "float32", 230, 5)]
     return [Job(i, k, d, s, n, arrival=0.0) for i, (k, d, s, n) in enumerate(spec)]
 
 
@@ -41,10 +42,10 @@ def test_each_job_is_the_same_alone_and_in_the_mix():
 
     from cutile.serving import StencilServer
 
-    print("serving: 9-job mix vs each job alone", flush=True)
+    print(f"serving: {len(_mix())}-job mix vs each job alone", flush=True)
     mix = StencilServer(keep_results=True)
     stats = mix.run(_mix(), realtime=False)
-    assert stats["jobs"] == 9
+    assert stats["jobs"] == len(_mix())
     for job in _mix():
         alone = StencilServer(keep_results=True)
         alone.run([job], realtime=False)
@@ -57,28 +58,31 @@ def test_float64_jobs_match_the_oracle(oracle):
     import torch
 
     from cutile import workloads as w
-    from cutile.serving import Job, StencilServer, _Class
+    from cutile.serving import SOLO_MIN_SIDE, Job, StencilServer, _Class
 
     print("serving: float64 jobs vs oracle", flush=True)
+    # job 3 is stepped alone with depth-2 temporal launches (two of them,
+    # then one sweep for its fifth step)
     jobs = [Job(0, "heat2d", "float64", 90, 11), Job(1, "diffusion3d", "float64", 26, 7),
-            Job(2, "grayscott", "float64", 61, 9)]
+            Job(2, "grayscott", "float64", 61, 9), Job(3, "diffusion3d", "float64", SOLO_MIN_SIDE, 5)]
     srv = StencilServer(keep_results=True)
     srv.run([Job(j.id, j.kind, j.dtype, j.side, j.steps) for j in jobs], realtime=False)
     for j in jobs:
         cls = _Class(j.kind, j.dtype)
         cls.allocate(j)                              # the same seeded initial fields
-        n = j.side
-        logical = tuple(slice(0, n + 2 * h) for h in cls.halo)
+        assert j.solo == (j.id == 3)
+        n, halo = j.side, cls.halo_of(j)
+        logical = tuple(slice(0, n + 2 * h) for h in halo)
         init = {f: to_host(j.bufs[f][0])[logical].copy() for f in cls.fields}
         if j.kind == "heat2d":
-            ref = {"u": oracle.time_march(init["u"], w.heat2d()._fn, 2, cls.halo, j.steps, w.dirichlet_spec(2))}
+            ref = {"u": oracle.time_march(init["u"], w.heat2d()._fn, 2, halo, j.steps, w.dirichlet_spec(2))}
         elif j.kind == "diffusion3d":
-            ref = {"u": oracle.time_march(init["u"], w.diffusion7(coef=0.1)._fn, 3, cls.halo, j.steps,
+            ref = {"u": oracle.time_march(init["u"], w.diffusion7(coef=0.1)._fn, 3, halo, j.steps,
                                           w.dirichlet_spec(3))}
         else:
-            ref = oracle.time_march_multi(init, [g._fn for g in w.gray_scott()], ["u", "v"], 2, cls.halo, j.steps,
+            ref = oracle.time_march_multi(init, [g._fn for g in w.gray_scott()], ["u", "v"], 2, halo, j.steps,
                                           w.periodic_spec(2))
-        I = oracle.interior(init["u"].shape, cls.halo)
+        I = oracle.interior(init["u"].shape, halo)
         for f in cls.fields:
             got = to_host(srv.results[j.id][f])[logical]
             assert np.array_equal(got[I], ref[f][I]), (j.kind, f)
