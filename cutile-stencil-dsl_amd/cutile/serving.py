@@ -61,7 +61,7 @@ class Job:
     keep: list = field(default_factory=list, repr=False)   # staging the device still reads
     recs: list = field(default_factory=list, repr=False)   # launch records, both parities
     fdesc: Optional[list] = field(default=None, repr=False)
-    solo: bool = False            # stepped alone, two steps per launch (large 3-D jobs)
+    solo: bool = False            # stepped alone, several steps per launch (large jobs)
     cur: int = 0                  # solo jobs: the buffer holding the current state
 
     @property
@@ -98,25 +98,29 @@ def generate_jobs(rate: float = 200.0, seconds: float = 60.0, seed: int = 0, fp3
     return jobs
 
 
-# 3-D jobs with at least this side are stepped alone, two steps per launch
-# (the register/LDS temporal kernel) instead of through the batched sweep:
-# on MI355X a depth-2 launch on one array advances 430-830 Gcell/s from side
-# 191 to 383 (FP32 and FP64, aligned rows or not) against 230-540 for the
-# batched sweep, and from this side up such jobs carry ~80 % of the 3-D work.
-SOLO_MIN_SIDE = 224
+# Jobs of these kinds with at least the given side are stepped alone, T
+# steps per launch (the register/LDS temporal kernel, zero Dirichlet), instead
+# of through the batched sweep.  On MI355X a depth-2 launch on one 3-D array
+# advances 430-830 Gcell/s from side 191 to 383 (FP32 and FP64, rows aligned
+# or not) against 230-540 for the batched sweep; a depth-4 launch on one 2-D
+# heat array 320-1030 from side 1531 to 4096 against 250-600.  From these
+# sides up the jobs carry ~80 % of their kind's work.
+SOLO = {"diffusion3d": (224, 2), "heat2d": (1536, 4)}     # kind -> (smallest side, steps per launch)
+SOLO_MIN_SIDE = SOLO["diffusion3d"][0]
 
 
 def warmup_jobs() -> List[Job]:
-    """Two-step jobs covering every class's kernels (a small job of every
-    (kind, storage) class and a solo-sized 3-D job per storage, three steps:
-    one depth-2 launch and one sweep), run before a timed stream so that no
+    """Jobs covering every class's kernels (a two-step small job of every
+    (kind, storage) class and a solo-sized job per solo kind and storage,
+    one temporal launch and one sweep), run before a timed stream so that no
     kernel is built while the clock runs."""
     out = []
     for kind in KINDS:
         for dtype in ("float32", "float64"):
             out.append(Job(len(out), kind, dtype, 40 if kind == "diffusion3d" else 128, 2))
-    for dtype in ("float32", "float64"):
-        out.append(Job(len(out), "diffusion3d", dtype, SOLO_MIN_SIDE, 3))
+    for kind, (side, depth) in SOLO.items():
+        for dtype in ("float32", "float64"):
+            out.append(Job(len(out), kind, dtype, side, depth + 1))
     return out
 
 
@@ -155,16 +159,17 @@ class _Class:
         self.tick = 0
         self.stream = None            # the class's own stream in the server
         self.solo = None
-        if kind == "diffusion3d":
+        if kind in SOLO:
             from cutile.runtime.timestep import TimeStepper
 
-            # zero-Dirichlet depth-2 blocks: the same states as two sweeps
-            self.solo = TimeStepper(fn, w.dirichlet_spec(3), temporal_steps=2)
-            if self.solo.T != 2:
-                raise RuntimeError("the 3-D serving class needs depth-2 temporal blocking")
+            # zero-Dirichlet blocks of T steps: the same states as T sweeps
+            self.solo_min, depth = SOLO[kind]
+            self.solo = TimeStepper(fn, w.dirichlet_spec(self.ndim), temporal_steps=depth)
+            if self.solo.T != depth or self.solo.block_reps != 1:
+                raise RuntimeError(f"serving class {kind}: no one-launch temporal kernel of depth {depth}")
 
     def is_solo(self, side: int) -> bool:
-        return self.solo is not None and side >= SOLO_MIN_SIDE
+        return self.solo is not None and side >= self.solo_min
 
     def shape(self, side: int) -> tuple:
         if self.is_solo(side):                           # the temporal kernel's own layout
@@ -251,8 +256,8 @@ class _Class:
 
     def step(self, stream) -> None:
         """Advance every active job of the class: batched jobs by one step
-        (one launch for all), solo jobs by two steps (one depth-2 launch
-        each; one sweep for a job's last odd step)."""
+        (one launch for all), solo jobs by T steps (one temporal launch each;
+        single sweeps for a job's last steps short of a block)."""
         from cutile.runtime import rt
 
         flip = self.tick % 2
@@ -268,9 +273,9 @@ class _Class:
                 j.done += 1
                 continue
             a, b = j.bufs["u"][j.cur], j.bufs["u"][1 - j.cur]
-            if j.steps - j.done >= 2:
+            if j.steps - j.done >= self.solo.T:
                 self.solo._block.launch_temporal([a], [b], self.solo.halo, stream, mode=0, validate=False)
-                j.done += 2
+                j.done += self.solo.T
             else:
                 self.solo._single.launch([a], [b], self.solo.halo, stream, validate=False)
                 j.done += 1

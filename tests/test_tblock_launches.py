@@ -138,19 +138,22 @@ def test_bench_launch_in_bounds(which, dtype, T, n, wrap):
 
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
-@pytest.mark.parametrize("side", [224, 231, 383, 384])
-def test_serving_solo_launch_in_bounds(dtype, side):
-    """The depth-2 launches the serving loop makes for large 3-D jobs
-    (zero Dirichlet, rows aligned or ragged: vector or scalar variant)."""
-    from cutile.serving import SOLO_MIN_SIDE
+@pytest.mark.parametrize("kind,side", [("diffusion3d", 224), ("diffusion3d", 231), ("diffusion3d", 383),
+                                       ("diffusion3d", 384), ("heat2d", 1536), ("heat2d", 1537), ("heat2d", 4095),
+                                       ("heat2d", 4096)])
+def test_serving_solo_launch_in_bounds(dtype, kind, side):
+    """The temporal launches the serving loop makes for large jobs (zero
+    Dirichlet, rows aligned or ragged: vector or scalar variant)."""
+    from cutile.serving import SOLO
 
-    assert side >= SOLO_MIN_SIDE
-    fn = w.diffusion7(dtype, coef=0.1)
-    r = compile(fn, temporal_steps=2, temporal_mode="zero")
+    lo, T = SOLO[kind]
+    assert side >= lo
+    fn = w.diffusion7(dtype, coef=0.1) if kind == "diffusion3d" else w.heat2d(dtype)
+    r = compile(fn, temporal_steps=T, temporal_mode="zero")
     halo = r.halo_widths
     shape = tuple(side + 2 * h for h in halo)
     kern, key, args = _launch_geometry(r, shape, r.storage)
     meta, cfg = _cfg_for(fn, r, key, r.storage)
-    got, errs = tm.run(fn._fn, 3, cfg, plan_accesses(meta), tblock_queue_range(meta), np.zeros(shape), halo,
-                       args.xch, tmode=0, wrap=0, edges=True)
+    got, errs = tm.run(fn._fn, len(shape), cfg, plan_accesses(meta), tblock_queue_range(meta), np.zeros(shape),
+                       halo, args.xch, tmode=0, wrap=0, edges=True)
     assert not errs, errs[:5]
