@@ -98,15 +98,20 @@ def generate_jobs(rate: float = 200.0, seconds: float = 60.0, seed: int = 0, fp3
     return jobs
 
 
-# Jobs of these kinds with at least the given side are stepped alone, T
-# steps per launch (the register/LDS temporal kernel, zero Dirichlet), instead
-# of through the batched sweep.  On MI355X a depth-2 launch on one 3-D array
-# advances 430-830 Gcell/s from side 191 to 383 (FP32 and FP64, rows aligned
-# or not) against 230-540 for the batched sweep; a depth-4 launch on one 2-D
-# heat array 320-1030 from side 1531 to 4096 against 250-600.  From these
-# sides up the jobs carry ~80 % of their kind's work.
-SOLO = {"diffusion3d": (224, 2), "heat2d": (1536, 4)}     # kind -> (smallest side, steps per launch)
-SOLO_MIN_SIDE = SOLO["diffusion3d"][0]
+# Jobs of these classes with at least the given side are stepped alone, T
+# steps per launch (the register/LDS temporal kernel, zero Dirichlet),
+# instead of through the batched sweep.  The sides are where the single-array
+# temporal launch overtakes the batched sweep on MI355X: a depth-2 launch on
+# one 3-D array advances 320-830 Gcell/s from side 191 to 383 against 230-540
+# for the batched sweep (FP64 gains from ~180, FP32 from ~220); a depth-4
+# launch on one 2-D heat array 320-1030 from side 1531 to 4096 against
+# 250-600 (FP64 gains from ~1300, FP32 from ~2000).
+SOLO = {                      # (kind, storage) -> (smallest side, steps per launch)
+    ("diffusion3d", "float32"): (224, 2),
+    ("diffusion3d", "float64"): (176, 2),
+    ("heat2d", "float32"): (2048, 4),
+    ("heat2d", "float64"): (1280, 4),
+}
 
 
 def warmup_jobs() -> List[Job]:
@@ -118,9 +123,8 @@ def warmup_jobs() -> List[Job]:
     for kind in KINDS:
         for dtype in ("float32", "float64"):
             out.append(Job(len(out), kind, dtype, 40 if kind == "diffusion3d" else 128, 2))
-    for kind, (side, depth) in SOLO.items():
-        for dtype in ("float32", "float64"):
-            out.append(Job(len(out), kind, dtype, side, depth + 1))
+    for (kind, dtype), (side, depth) in SOLO.items():
+        out.append(Job(len(out), kind, dtype, side, depth + 1))
     return out
 
 
@@ -159,11 +163,11 @@ class _Class:
         self.tick = 0
         self.stream = None            # the class's own stream in the server
         self.solo = None
-        if kind in SOLO:
+        if (kind, dtype) in SOLO:
             from cutile.runtime.timestep import TimeStepper
 
             # zero-Dirichlet blocks of T steps: the same states as T sweeps
-            self.solo_min, depth = SOLO[kind]
+            self.solo_min, depth = SOLO[(kind, dtype)]
             self.solo = TimeStepper(fn, w.dirichlet_spec(self.ndim), temporal_steps=depth)
             if self.solo.T != depth or self.solo.block_reps != 1:
                 raise RuntimeError(f"serving class {kind}: no one-launch temporal kernel of depth {depth}")

@@ -137,17 +137,22 @@ def test_bench_launch_in_bounds(which, dtype, T, n, wrap):
     assert not errs, errs[:5]
 
 
-@pytest.mark.parametrize("dtype", ["float32", "float64"])
-@pytest.mark.parametrize("kind,side", [("diffusion3d", 224), ("diffusion3d", 231), ("diffusion3d", 383),
-                                       ("diffusion3d", 384), ("heat2d", 1536), ("heat2d", 1537), ("heat2d", 4095),
-                                       ("heat2d", 4096)])
-def test_serving_solo_launch_in_bounds(dtype, kind, side):
-    """The temporal launches the serving loop makes for large jobs (zero
-    Dirichlet, rows aligned or ragged: vector or scalar variant)."""
+def _solo_cases():
     from cutile.serving import SOLO
 
-    lo, T = SOLO[kind]
-    assert side >= lo
+    out = []
+    for (kind, dtype), (lo, T) in SOLO.items():
+        top = 384 if kind == "diffusion3d" else 4096
+        for side in (lo, lo + 1, top - 1, top):
+            out.append(pytest.param(kind, dtype, side, T, id=f"{kind}-{dtype}-{side}"))
+    return out
+
+
+@pytest.mark.parametrize("kind,dtype,side,T", _solo_cases())
+def test_serving_solo_launch_in_bounds(kind, dtype, side, T):
+    """The temporal launches the serving loop makes for large jobs (zero
+    Dirichlet, rows aligned or ragged: vector or scalar variant), from the
+    smallest solo side to the largest side of the job stream."""
     fn = w.diffusion7(dtype, coef=0.1) if kind == "diffusion3d" else w.heat2d(dtype)
     r = compile(fn, temporal_steps=T, temporal_mode="zero")
     halo = r.halo_widths
