@@ -18,6 +18,10 @@ step); this is the request loop of the serving configuration:
   their tiles, with per-size-class tile buckets; periodic halos of a class
   are refilled by one batched fill launch per axis; Dirichlet halos are
   zero once and stay so (the sweep writes interiors only);
+* **solo temporal stepping**: large 3-D and 2-D heat jobs (``SOLO``) are
+  stepped alone, several steps per launch of the register/LDS temporal
+  kernel, where that outruns the batched sweep; their last steps short of a
+  block run as single sweeps;
 * **continuous batching**: jobs join at the next tick after they are
   admitted and leave after their last step, so work of jobs that arrived at
   different times overlaps on the device; the work-lists are rebuilt only
@@ -29,7 +33,8 @@ step); this is the request loop of the serving configuration:
 
 Each job's result depends only on its own fields: the batched kernel
 evaluates every instance with the same arithmetic as the single-instance
-kernel, so a job gives the same bits alone and in any mix.
+kernel, and a temporal block gives the same bits as its T sweeps, so a job
+gives the same bits alone and in any mix.
 """
 
 from __future__ import annotations
