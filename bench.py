@@ -259,19 +259,26 @@ def gray_scott_c3(n=4096, steps=1000):
     the 64 seeded squares die out: v decays to ~1e-41 (subnormal) by step
     1000 and u tends to 1, so the final errors are small for u and measure
     subnormal resolution for v; the errors at step 200, while the pattern is
-    alive, are the informative precision comparison."""
+    alive, are the informative precision comparison.  float64 and float32
+    also run through the coupled-field temporal kernel (``_T2``, ``_T4``:
+    blocks of 2 or 4 fused steps per launch), whose final state must equal
+    the one-step run of the same precision bit for bit."""
     import torch
 
     from cutile import compile_fused
     from cutile.runtime import boundary, rt
+    from cutile.runtime.launcher import compile_fused_block
     from cutile.runtime.timestep import FusedStepper
     from cutile.workloads import GS, gray_scott, gray_scott_init, gray_scott_rates, periodic_spec
 
     spec = periodic_spec(2)
     r64 = compile_fused(list(gray_scott("float64")))
+    r32 = compile_fused(list(gray_scott("float32")))
+    blocks = {(dt, T): compile_fused_block(list(gray_scott(dt)), T, r) for dt, r in (("float64", r64), ("float32", r32))
+              for T in (2, 4)}
     variants = [
         ("float64", r64, None, torch.float64),
-        ("float32", compile_fused(list(gray_scott("float32"))), None, torch.float32),
+        ("float32", r32, None, torch.float32),
         ("bfloat16_kahan", compile_fused(list(gray_scott_rates("float32")), update="kahan", dt=GS["dt"],
                                          targets=["u", "v"], storage="bfloat16"), "kahan", torch.bfloat16),
         ("float32_kahan", compile_fused(list(gray_scott_rates("float32")), update="kahan", dt=GS["dt"],
@@ -279,9 +286,14 @@ def gray_scott_c3(n=4096, steps=1000):
         # float64 runs from the initial state rounded to float32 / bfloat16
         ("float64_from_float32_init", r64, None, torch.float32),
         ("float64_from_bfloat16_init", r64, None, torch.bfloat16),
-    ]
+    ] + [(f"{dt}_T{T}", r64 if dt == "float64" else r32, None, getattr(torch, dt)) for dt in ("float64", "float32")
+         for T in (2, 4)]
     out, finals, early = {}, {}, {}
     for name, r, upd, init_dt in variants:
+        blk = None
+        if "_T" in name:
+            bdt, bT = name.split("_T")
+            blk = blocks[bdt, int(bT)]
         u0, v0 = gray_scott_init(n, r.halo_widths, seed=0)
         tdt = getattr(torch, r.storage)
         f = {"u": torch.from_numpy(u0).cuda().to(init_dt).to(tdt), "v": torch.from_numpy(v0).cuda().to(init_dt).to(tdt)}
@@ -289,13 +301,17 @@ def gray_scott_c3(n=4096, steps=1000):
             f["c_u"], f["c_v"] = torch.zeros_like(f["u"]), torch.zeros_like(f["v"])
         targets = list(r.outputs) if upd else ["u", "v"]
         scratch = {t: torch.empty_like(f[t]) for t in targets}
-        stp = FusedStepper(r, spec, targets)
+        stp = FusedStepper(r, spec, targets, block=blk)
         for t in targets:
             boundary.fill(f[t], spec, r.halo_widths)
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
-            stp.enqueue(f, scratch, 2, side.cuda_stream)    # JIT (even count: buffers come back)
+            stp.enqueue(f, scratch, 2, side.cuda_stream)    # JIT of the one-step kernel
+            if blk is not None:                              # and of the block kernel, before any capture
+                bk = blk.load_module()._KERNEL
+                bk._function(bk.temporal_key([f[t] for t in r.inputs], [scratch[t] for t in targets],
+                                             r.halo_widths))
             # two graphs: 198 steps (to a snapshot at step 200, while the
             # pattern is alive) and 200 steps (replayed to the end).  The
             # wrapped sweeps leave halos stale; each graph refills them once,
@@ -329,6 +345,13 @@ def gray_scott_c3(n=4096, steps=1000):
     def err(a, b, t):
         return float((a[t] - b[t]).abs().max() / b[t].abs().max())
 
+    for dt in ("float64", "float32"):
+        for T in (2, 4):
+            name = f"{dt}_T{T}"
+            same = all(torch.equal(finals[name][t], finals[dt][t]) for t in ("u", "v"))
+            out[name]["state_equals_T1"] = bool(same)
+            out[name]["kernel"] = f"coupled-field temporal kernel, {T} fused steps per launch"
+            log(f"gray-scott {name}: final state equal to the one-step run: {same}")
     ref = finals["float64"]
     out["float64"]["max_abs_u"] = float(ref["u"].abs().max())
     out["float64"]["max_abs_v"] = float(ref["v"].abs().max())

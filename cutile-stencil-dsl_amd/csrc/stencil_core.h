@@ -560,7 +560,8 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   constexpr int D = QHI + 1;
   constexpr int GY = P::GY, GZ = P::GZ, OTY = TY - 2 * GY, OTZ = TZ - 2 * GZ;
   constexpr int ES = (int)sizeof(S);
-  static_assert(P::NF == 1 && P::NO == 1 && P::UPD == 0, "temporal blocking: one field, one result");
+  constexpr int NF = P::NF;
+  static_assert(P::NO == NF && P::UPD == 0, "temporal blocking: result m is the next state of field m");
   static_assert(T::HASQ && QLO <= 0 && QHI >= 0, "level queues must hold plane offset 0");
   static_assert(OTY > 0 && OTZ > 0 && GZ % VZ == 0 && TZ % VZ == 0, "temporal tile geometry");
   static_assert(TT >= 2, "at least two levels");
@@ -618,20 +619,20 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     }
   }
 
-  // plane x of the input: its index in the array, and whether it lies there
-  auto in_plane = [&](int x, bool& ok) -> Rsrc {
+  // plane x of input field f: its index in the array, and whether it lies there
+  auto in_plane = [&](int f, int x, bool& ok) -> Rsrc {
     const int px = w0 ? wrapm(x, a.n0) : x;
     ok = a.h0 + px >= 0 && a.h0 + px < a.s0;
-    return make_rsrc(reinterpret_cast<const S*>(a.in[0]) + (i64)(a.h0 + (ok ? px : xb)) * plane, pbytes);
+    return make_rsrc(reinterpret_cast<const S*>(a.in[f]) + (i64)(a.h0 + (ok ? px : xb)) * plane, pbytes);
   };
 
-  C q[TT][QN][KC][VZ];           // level l's queue (level 0: the input)
-  C pq[KC][VZ];                  // level-0 plane prefetched for the next step
+  C q[TT][NF][QN][KC][VZ];       // level l's queue of every field (level 0: the inputs)
+  C pq[NF][KC][VZ];              // level-0 planes prefetched for the next step
   C pb[NB > 0 ? NB : 1][NVB][VZ];  // level-0 halo ring prefetched for the next step
 
-  auto load_q0 = [&](int x, C (&dst)[KC][VZ], bool live) {
+  auto load_q0 = [&](int f, int x, C (&dst)[KC][VZ], bool live) {
     bool ok;
-    const Rsrc r = in_plane(x, ok);
+    const Rsrc r = in_plane(f, x, ok);
     ok = ok && live;
 #pragma unroll
     for (int k = 0; k < KC; ++k) {
@@ -644,7 +645,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       bool ok;
-      const Rsrc r = in_plane(x + P::BDX[b], ok);
+      const Rsrc r = in_plane(P::BF[b], x + P::BDX[b], ok);
       ok = ok && live;
 #pragma unroll
       for (int j = 0; j < NVB; ++j) {
@@ -663,20 +664,23 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   //      prefetched; levels >= 1 and the LDS start at 0 (the halo rings of
   //      levels >= 1 are never written)
 #pragma unroll
-  for (int i = 0; i + 1 < QN; ++i) load_q0(s_first + QLO + i, q[0][i], true);
-  load_q0(s_first + QHI, pq, true);
+  for (int f = 0; f < NF; ++f) {
+#pragma unroll
+    for (int i = 0; i + 1 < QN; ++i) load_q0(f, s_first + QLO + i, q[0][f][i], true);
+    load_q0(f, s_first + QHI, pq[f], true);
+  }
   load_ring(s_first, true);
 #pragma unroll
   for (int l = 1; l < TT; ++l)
 #pragma unroll
-    for (int i = 0; i < QN; ++i)
+    for (int f = 0; f < NF; ++f)
 #pragma unroll
-      for (int k = 0; k < KC; ++k)
+      for (int i = 0; i < QN; ++i)
 #pragma unroll
-        for (int v = 0; v < VZ; ++v) q[l][i][k][v] = C(0);
+        for (int k = 0; k < KC; ++k)
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) q[l][f][i][k][v] = C(0);
   for (int i = tid; i < TT * LE; i += NT) lds[i] = C(0);
-
-  S* out = reinterpret_cast<S*>(a.out[0]);
 
   // one step; R = (s - s_first) % QN.  Plane p of level l sits in queue
   // slot (p - QLO - s_first) mod QN.
@@ -684,9 +688,11 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     constexpr int R = decltype(Rc)::value;
     // level-0 head: plane s + QHI
 #pragma unroll
-    for (int k = 0; k < KC; ++k)
+    for (int f = 0; f < NF; ++f)
 #pragma unroll
-      for (int v = 0; v < VZ; ++v) q[0][pmod(R - 1, QN)][k][v] = pq[k][v];
+      for (int k = 0; k < KC; ++k)
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) q[0][f][pmod(R - 1, QN)][k][v] = pq[f][k][v];
     __syncthreads();
     // LDS planes: level l's buffer b holds plane s - l*D + BDX[b] of level l
     for_each_ic([&](auto Lc) {
@@ -700,7 +706,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
           if (act[k]) {
             C* d = base + (P::BRY[b] + ly[k]) * T::pitch(b) + T::rzv(b) + lz[k];
 #pragma unroll
-            for (int v = 0; v < VZ; ++v) d[v] = q[L][sl][k][v];
+            for (int v = 0; v < VZ; ++v) d[v] = q[L][P::BF[b]][sl][k][v];
           }
         }
         if constexpr (L == 0) {
@@ -720,7 +726,8 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     }, typename mkseq<TT>::type{});
     __syncthreads();
     // prefetch the next step's level-0 plane and ring while this step computes
-    load_q0(s + 1 + QHI, pq, s + 1 <= s_last);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) load_q0(f, s + 1 + QHI, pq[f], s + 1 <= s_last);
     load_ring(s + 1, s + 1 <= s_last);
 
     // levels TT .. 1
@@ -732,7 +739,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       // there are never read, 0 is pushed)
       constexpr int WY = (TT - L) * P::RY, WZ = (TT - L) * P::RZ;
       const int p = s - (L - 1) * D;                  // plane evaluated
-      C res[KC][VZ];
+      C res[NF][KC][VZ];
 #pragma unroll
       for (int k = 0; k < KC; ++k) {
         const bool need = act[k] && ly[k] >= GY - WY && ly[k] < GY + OTY + WY &&
@@ -740,25 +747,32 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
         if (__builtin_amdgcn_ballot_w64(need) != 0) {
 #pragma unroll
           for (int v = 0; v < VZ; ++v) {
-            Cell<P, RL, KC> c{&q[L - 1], lds + (L - 1) * LE, k, v, ly[k], lz[k] + v};
-            C o[1];
+            Cell<P, RL, KC> c{q[L - 1], lds + (L - 1) * LE, k, v, ly[k], lz[k] + v};
+            C o[NF];
             P::eval(c, o);
-            res[k][v] = o[0];
+#pragma unroll
+            for (int m = 0; m < NF; ++m) res[m][k][v] = o[m];
           }
         } else {
 #pragma unroll
-          for (int v = 0; v < VZ; ++v) res[k][v] = C(0);
+          for (int m = 0; m < NF; ++m)
+#pragma unroll
+            for (int v = 0; v < VZ; ++v) res[m][k][v] = C(0);
         }
       }
       if constexpr (L == TT) {
         const bool st = p >= xb;
-        const Rsrc ro = make_rsrc(out + (i64)(a.h0 + (st ? p : xb)) * plane, pbytes);
 #pragma unroll
-        for (int k = 0; k < KC; ++k) {
-          Vec<S, VZ> w;
+        for (int m = 0; m < NF; ++m) {
+          S* out = reinterpret_cast<S*>(a.out[m]);
+          const Rsrc ro = make_rsrc(out + (i64)(a.h0 + (st ? p : xb)) * plane, pbytes);
 #pragma unroll
-          for (int v = 0; v < VZ; ++v) w.v[v] = Cvt<S, C>::st(res[k][v]);
-          bstore<S, VZ, P::NTS ? BUF_NT : 0>(w, ro, st ? soff[k] : BUF_OOB);
+          for (int k = 0; k < KC; ++k) {
+            Vec<S, VZ> w;
+#pragma unroll
+            for (int v = 0; v < VZ; ++v) w.v[v] = Cvt<S, C>::st(res[m][k][v]);
+            bstore<S, VZ, P::NTS ? BUF_NT : 0>(w, ro, st ? soff[k] : BUF_OOB);
+          }
         }
       } else {
         constexpr int sl = pmod(R - (L - 1) * D - QLO, QN);
@@ -767,7 +781,9 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
         for (int k = 0; k < KC; ++k) {
           const bool zero = zero_out && !(pin && keep[k]);
 #pragma unroll
-          for (int v = 0; v < VZ; ++v) q[L][sl][k][v] = zero ? C(0) : res[k][v];
+          for (int m = 0; m < NF; ++m)
+#pragma unroll
+            for (int v = 0; v < VZ; ++v) q[L][m][sl][k][v] = zero ? C(0) : res[m][k][v];
         }
       }
     }, typename mkseq<TT>::type{});

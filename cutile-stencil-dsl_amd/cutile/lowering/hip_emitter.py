@@ -184,7 +184,8 @@ def tblock_estimate(meta: StencilMeta, cfg: TBlockConfig, storage: str, compute:
     # queues of T levels + prefetched plane and ring + per-cell offsets and
     # flags + the expression's working set (its loaded operands)
     work = cw * (min(len(access_table(meta)), 40) + 8) + 24
-    regs = cw * cfg.vz * cfg.kc * (cfg.T * qn + 1) + cw * cfg.vz * nb * nvb + 16 * cfg.kc + nb * nvb + work
+    nf = max(1, meta.num_inputs)
+    regs = cw * cfg.vz * cfg.kc * nf * (cfg.T * qn + 1) + cw * cfg.vz * nb * nvb + 16 * cfg.kc + nb * nvb + work
     return {"lds": lds, "vgpr": regs, "qn": qn}
 
 
@@ -198,7 +199,8 @@ def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
     LDS-ring kernel.  ``force`` = (output rows, output columns, threads)
     restricts the search to that shape (a tuning hint; it must satisfy the
     same limits)."""
-    if meta.num_inputs != 1 or meta.num_exprs != 1 or meta.update or T < 2:
+    # one field, or coupled fields whose result m is the next state of field m
+    if meta.num_exprs != meta.num_inputs or meta.update or T < 2:
         return None
     rx, ry, rz = radius3(meta)
     nd = meta.ndim

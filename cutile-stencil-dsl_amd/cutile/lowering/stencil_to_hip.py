@@ -127,8 +127,9 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
     tblocks = []
     tblock_depth = 0
     if temporal_steps > 1:
-        if meta.num_inputs != 1 or meta.num_outputs != 1:
-            raise NotImplementedError("in-kernel temporal blocking needs one input and one output")
+        if meta.num_outputs != meta.num_inputs or meta.num_exprs != meta.num_inputs or meta.update:
+            raise NotImplementedError("in-kernel temporal blocking needs one result per field "
+                                      "(result m is the next state of field m)")
         # register/LDS kernel at the vector width and at scalar width (arrays
         # whose rows are not vector-aligned); the LDS-ring kernel when the
         # ghost ring of T levels does not fit the registers or the LDS
@@ -143,6 +144,8 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
         if tblocks:
             qlo, qhi = tblock_queue_range(meta)
             tblock_depth = int(temporal_steps) * (qhi - qlo + 1)
+        elif meta.num_inputs != 1:
+            raise NotImplementedError("coupled fields at this temporal depth fit neither the registers nor the LDS")
         else:
             ty, tz = temporal_tile(meta.ndim, temporal_steps, radius3(meta), compute_t)
             temporal = (KernelConfig(ty=ty, tz=tz, vz=1, nt=256), int(temporal_steps))
@@ -280,10 +283,13 @@ def lower_stencil_to_python(module, domain=None, tile_sizes=None, halo_widths=No
 def lower_fused_stencils_to_python(modules: Sequence, tile_sizes=None, halo_widths=None,
                                    temporal_steps: int = 1, kernel_hints=None,
                                    spatial_term_order=None, storage=None, update=None, dt: float = 1.0,
-                                   targets=None) -> str:
-    if temporal_steps != 1:
-        raise NotImplementedError(
-            "temporal blocking of fused kernels is not supported (temporal_steps must be 1)")
+                                   targets=None, temporal_mode: str = "zero") -> str:
+    """Module of a fused kernel.  ``temporal_steps > 1`` (reached through
+    ``FusedStepper``; ``compile_fused`` keeps the upstream restriction to
+    one step) needs a closed system: member m's result is the next state of
+    input m."""
+    if temporal_steps != 1 and update is not None:
+        raise NotImplementedError("a time-update epilogue runs one step per launch (temporal_steps=1)")
     meta = extract_fused_meta(modules)
     if update is not None:
         apply_update(meta, update, dt, targets)
@@ -294,4 +300,5 @@ def lower_fused_stencils_to_python(modules: Sequence, tile_sizes=None, halo_widt
 
         tp = TilingPass(dtype_bytes=4 if meta.dtype == "float32" else 8)
         tile_sizes = tuple(tp.tile_for(meta.ndim, len(meta.accesses)))
-    return emit_module(meta, tile_sizes, halo_widths, 1, None, kernel_hints, storage, fused=True)
+    return emit_module(meta, tile_sizes, halo_widths, temporal_steps, None, kernel_hints, storage, fused=True,
+                       temporal_mode=temporal_mode)

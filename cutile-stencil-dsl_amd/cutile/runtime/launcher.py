@@ -467,3 +467,36 @@ def compile_fused(stencil_fns: list, domain=None, hw=None, tile_sizes=None, halo
         outputs=tuple(meta.output_names), dtype=dtype, storage=store, hip_source=src,
         _ref_fns=[fn._fn for fn in stencil_fns], _ref_inputs=meta.stencil_inputs,
     )
+
+
+def compile_fused_block(stencil_fns: list, temporal_steps: int, like: CompileResult) -> CompileResult:
+    """Temporal-blocking kernel of a fused closed system: member m's result
+    is the next state of input m (fused Gray-Scott: ``gs_u``, ``gs_v`` of
+    ``u``, ``v``), ``temporal_steps`` steps per launch with the halo, tile
+    and storage of the one-step result ``like``.  Used by ``FusedStepper``;
+    ``compile_fused`` keeps the upstream restriction to one step
+    (``launcher.py:652-655``), this goes beyond it.  The launch semantics
+    are the single-field kernel's (``launch_temporal``: zero-halo levels, or
+    periodic wrap), so a block equals ``temporal_steps`` fused steps bit for
+    bit under zero-Dirichlet or periodic boundaries."""
+    from cutile.lowering.meta import extract_fused_meta
+    from cutile.lowering.stencil_to_hip import lower_fused_stencils_to_python
+
+    modules = [fn._ir for fn in stencil_fns]
+    meta = extract_fused_meta(modules)
+    if list(meta.input_names) != list(like.inputs) or len(meta.output_names) != len(meta.input_names):
+        raise ValueError("a fused temporal block needs one member per input, member m advancing input m")
+    T = int(temporal_steps)
+    if T < 2:
+        raise ValueError("temporal_steps must be at least 2")
+    store = like.storage if like.storage != like.dtype else None
+    code = lower_fused_stencils_to_python(modules, tile_sizes=like.tile_sizes, halo_widths=like.halo_widths,
+                                          temporal_steps=T, storage=store, temporal_mode="zero")
+    src = code.split("_HIP_SOURCE = r'''\n", 1)[1].split("'''", 1)[0]
+    return CompileResult(
+        name=meta.name, code=code, ndim=meta.ndim, halo_widths=tuple(like.halo_widths),
+        tile_sizes=tuple(like.tile_sizes), temporal_steps=T, analysis=dict(like.analysis),
+        stencil_halo=like.stencil_halo, kernel_hints={}, resources=None, inputs=tuple(meta.input_names),
+        outputs=tuple(meta.output_names), dtype=like.dtype, storage=like.storage, hip_source=src,
+        temporal_mode="zero", _ref_fns=[fn._fn for fn in stencil_fns], _ref_inputs=meta.stencil_inputs,
+    )

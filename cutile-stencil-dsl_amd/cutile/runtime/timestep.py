@@ -243,7 +243,8 @@ class FusedStepper:
     ``targets=result.outputs`` and ``fill`` = the fields whose halo the
     stencil reads (compensations are read at the centre only)."""
 
-    def __init__(self, result, boundary, targets: Sequence[str], fill: Optional[Sequence[str]] = None):
+    def __init__(self, result, boundary, targets: Sequence[str], fill: Optional[Sequence[str]] = None,
+                 block=None):
         self.r = result
         self.spec = boundary
         self.targets = list(targets)
@@ -251,6 +252,18 @@ class FusedStepper:
         self.halo = result.halo_widths
         self._k = result.load_module()._KERNEL
         self.policy = HaloPolicy(boundary, result.ndim, self.halo)
+        # ``block``: ``compile_fused_block`` of the same members, T steps per
+        # launch.  Exact for a closed system (targets = inputs, in order)
+        # under periodic or zero-Dirichlet boundaries only
+        self._blk, self.T = None, 1
+        if block is not None:
+            kind = _bc_kind(boundary, result.ndim)
+            if self.targets != list(result.inputs) or list(block.inputs) != list(result.inputs):
+                raise ValueError("a fused temporal block needs targets equal to the inputs, in order")
+            if tuple(block.halo_widths) != tuple(self.halo):
+                raise ValueError("the fused temporal block must share the one-step kernel's halo")
+            if kind in ("periodic", "dirichlet0"):
+                self._blk, self.T = block.load_module()._KERNEL, int(block.temporal_steps)
 
     def enqueue(self, fields: dict, scratch: dict, steps: int, stream=None):
         from cutile.runtime.kernel import default_stream
@@ -261,6 +274,19 @@ class FusedStepper:
             for t in self.fill_names:
                 _boundary.fill(scratch[t], self.spec, self.halo, stream)
         axes = self.policy.axes_for(fields[self.targets[0]])
+        nblk = steps // self.T if self._blk is not None else 0
+        wrap = tuple(range(self.r.ndim)) if self.policy.mode == "wrap" else ()
+        for _ in range(nblk):
+            ins = [fields[n] for n in self.r.inputs]
+            outs = [scratch[t] for t in self.targets]
+            self._blk.launch_temporal(ins, outs, self.halo, stream, mode=0, validate=False, periodic_axes=wrap)
+            for t in self.targets:
+                fields[t], scratch[t] = scratch[t], fields[t]
+        blocked = nblk > 0
+        steps -= nblk * self.T
+        if blocked and wrap and steps and not axes:
+            for t in self.fill_names:     # single steps that read the halo follow wrapped blocks
+                _boundary.fill(fields[t], self.spec, self.halo, stream)
         for _ in range(steps):
             ins = [fields[n] for n in self.r.inputs]
             outs = [scratch[t] for t in self.targets]
@@ -270,7 +296,7 @@ class FusedStepper:
                     _boundary.fill(scratch[t], self.spec, self.halo, stream)
             for t in self.targets:
                 fields[t], scratch[t] = scratch[t], fields[t]
-        if axes and steps:
+        if (axes and steps) or (blocked and wrap):
             for t in self.fill_names:     # the wrapped sweeps left the halos stale
                 _boundary.fill(fields[t], self.spec, self.halo, stream)
         return fields, scratch

@@ -93,6 +93,25 @@ def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, ed
     workgroups of the first two and last two tiles / chunks along each axis
     (bounds of a large launch; the output is then partial).  Returns
     (output array, list of error strings)."""
+    outs, errs = run_fields([fn], [[0]], ndim, cfg, plan, qrange, [u], halo, xch, tmode, wrap,
+                            None if out is None else [out], edges, rad)
+    return outs[0], errs
+
+
+def _arg_fields(fn, ndim, names):
+    """Field index of each of ``fn``'s array parameters (by name)."""
+    code = fn.__code__
+    params = code.co_varnames[:code.co_argcount - ndim]
+    return [names.index(p) for p in params]
+
+
+def run_fields(fns, argmap, ndim, cfg, plan, qrange, us, halo, xch, tmode=0, wrap=0, outs=None, edges=False,
+               rad=None):
+    """``run`` for coupled fields: ``fns[m]`` gives the next state of field
+    m from the fields listed by ``argmap[m]`` (one field index per array
+    parameter; a single-field stencil: ``[0]``), ``us[f]`` is field f.
+    Returns (one output array per field, error strings)."""
+    NF = len(us)
     TT, TY, TZ, VZ, NT = cfg.T, cfg.ty, cfg.tz, cfg.vz, cfg.nt
     if rad is None:      # stencil radius in the 3-D map, from the LDS buffers and queue range
         rad = (max(-qrange[0], qrange[1]), max([b[2] for b in plan["bufs"]] + [0]),
@@ -101,23 +120,24 @@ def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, ed
     QLO, QHI = qrange
     QN, D = QHI - QLO + 1, QHI + 1
     errs = []
-    u3 = np.asarray(u).reshape(map3(ndim, u.shape))
+    u3s = [np.asarray(u).reshape(map3(ndim, u.shape)) for u in us]
+    u3 = u3s[0]
     s0, s1, s2 = u3.shape
     h0, h1, h2 = halo3(ndim, halo)
     n0, n1, n2 = s0 - 2 * h0, s1 - 2 * h1, s2 - 2 * h2
     w0, w1, w2 = bool(wrap & 1), bool(wrap & 2), bool(wrap & 4)
-    res = np.array(u3, copy=True) if out is None else np.asarray(out).reshape(u3.shape)
+    res = [np.array(x, copy=True) for x in u3s] if outs is None else [np.asarray(o).reshape(u3.shape) for o in outs]
     written = np.zeros(u3.shape, dtype=np.int32)
-    f = oracle.array_aware(fn)
+    fs = [oracle.array_aware(fn) for fn in fns]
     bufs = []
     for (fi, dx, ry, rz) in plan["bufs"]:
         rzv = -(-rz // VZ) * VZ
         cols = TZ + 2 * rzv
         w = cols // VZ
         sv = rzv // VZ
-        bufs.append(dict(dx=dx, ry=ry, rz=rz, rzv=rzv, rows=TY + 2 * ry, cols=cols, w=w, sv=sv, TY=TY,
+        bufs.append(dict(f=fi, dx=dx, ry=ry, rz=rz, rzv=rzv, rows=TY + 2 * ry, cols=cols, w=w, sv=sv, TY=TY,
                          gvecs=2 * ry * w + TY * 2 * sv))
-    bidx = {b["dx"]: k for k, b in enumerate(bufs)}
+    bidx = {(b["f"], b["dx"]): k for k, b in enumerate(bufs)}
     cells = TY * (TZ // VZ)
     if -(-cells // NT) != cfg.kc:
         errs.append("cells per thread")
@@ -148,7 +168,7 @@ def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, ed
         mine = (ly >= GY) & (ly < GY + OTY) & (lz >= GZ) & (lz < GZ + OTZ) & iny & inz
         vz_off = lz - lzv
 
-        def load_q0(x, live):
+        def load_q0(fi, x, live):
             px, ok = plane(x)
             ok = ok and live
             vals = np.zeros((TY, TZ), dtype=np.float64)
@@ -157,7 +177,7 @@ def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, ed
                 if m.any():
                     if (h2 + zq[m] + VZ > s2).any() or (h1 + yq[m] >= s1).any():
                         errs.append(f"queue load outside plane {x}")
-                    vals[m] = u3[h0 + px, h1 + yq[m], h2 + zq[m] + vz_off[m]]
+                    vals[m] = u3s[fi][h0 + px, h1 + yq[m], h2 + zq[m] + vz_off[m]]
             return vals
 
         # level-0 halo ring of every buffer: LDS (row, column) and in-plane
@@ -190,74 +210,81 @@ def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, ed
                 v = np.zeros((len(rr), VZ))
                 if pok and live and ok.any():
                     cols = gz[ok][:, None] + np.arange(VZ)[None, :]
-                    v[ok] = u3[h0 + px, gy[ok][:, None], cols]
+                    v[ok] = u3s[b["f"]][h0 + px, gy[ok][:, None], cols]
                 ring.append(v)
             return ring
 
         s_first = xb + (TT - 1) * QLO
         s_last = xe - 1 + (TT - 1) * D
-        q = [[np.zeros((TY, TZ)) for _ in range(QN)] for _ in range(TT)]
+        q = [[[np.zeros((TY, TZ)) for _ in range(QN)] for _ in range(NF)] for _ in range(TT)]
         lds = [[np.zeros((b["rows"], b["cols"])) for b in bufs] for _ in range(TT)]
-        for i in range(QN - 1):
-            q[0][i] = load_q0(s_first + QLO + i, True)
-        pq = load_q0(s_first + QHI, True)
+        for fi in range(NF):
+            for i in range(QN - 1):
+                q[0][fi][i] = load_q0(fi, s_first + QLO + i, True)
+        pq = [load_q0(fi, s_first + QHI, True) for fi in range(NF)]
         pb = load_ring(s_first, True)
         for s in range(s_first, s_last + 1):
             R = (s - s_first) % QN
-            q[0][_pmod(R - 1, QN)] = pq
+            for fi in range(NF):
+                q[0][fi][_pmod(R - 1, QN)] = pq[fi]
             for L in range(TT):
                 for k, b in enumerate(bufs):
                     sl = _pmod(R - L * D + b["dx"] - QLO, QN)
-                    lds[L][k][b["ry"]:b["ry"] + TY, b["rzv"]:b["rzv"] + TZ] = q[L][sl]
+                    lds[L][k][b["ry"]:b["ry"] + TY, b["rzv"]:b["rzv"] + TZ] = q[L][b["f"]][sl]
                     if L == 0:
                         rr, cc = rings[k][0], rings[k][1]
                         cols = cc[:, None] * VZ + np.arange(VZ)[None, :]
                         lds[0][k][rr[:, None], cols] = pb[k]
-            pq = load_q0(s + 1 + QHI, s + 1 <= s_last)
+            pq = [load_q0(fi, s + 1 + QHI, s + 1 <= s_last) for fi in range(NF)]
             pb = load_ring(s + 1, s + 1 <= s_last)
             for L in range(TT, 0, -1):
                 RL = _pmod(R - (L - 1) * D, QN)
                 p = s - (L - 1) * D
 
-                def get(dx, dy, dz, L=L, RL=RL):
+                def get(fi, dx, dy, dz, L=L, RL=RL):
                     if dy == 0 and dz == 0:
                         if not QLO <= dx <= QHI:
                             errs.append(f"queue read dx={dx} outside [{QLO}, {QHI}]")
                             return np.zeros((TY, TZ))
-                        return q[L - 1][_pmod(dx - QLO + RL, QN)]
-                    if dx not in bidx:
-                        errs.append(f"no LDS buffer for dx={dx}")
+                        return q[L - 1][fi][_pmod(dx - QLO + RL, QN)]
+                    if (fi, dx) not in bidx:
+                        errs.append(f"no LDS buffer for field {fi}, dx={dx}")
                         return np.zeros((TY, TZ))
-                    b = bufs[bidx[dx]]
+                    b = bufs[bidx[fi, dx]]
                     r0, c0 = b["ry"] + dy, b["rzv"] + dz
                     if r0 < 0 or r0 + TY > b["rows"] or c0 < 0 or c0 + TZ > b["cols"]:
-                        errs.append(f"LDS read ({dx},{dy},{dz}) outside buffer")
+                        errs.append(f"LDS read ({fi},{dx},{dy},{dz}) outside buffer")
                         return np.zeros((TY, TZ))
-                    return lds[L - 1][bidx[dx]][r0:r0 + TY, c0:c0 + TZ]
+                    return lds[L - 1][bidx[fi, dx]][r0:r0 + TY, c0:c0 + TZ]
 
-                val = np.broadcast_to(np.asarray(f(_Proxy(ndim, get), *([0] * ndim)), dtype=np.float64),
-                                      (TY, TZ))
+                vals = []
+                for fm, amap in zip(fs, argmap):
+                    proxies = [_Proxy(ndim, lambda dx, dy, dz, fi=fi: get(fi, dx, dy, dz)) for fi in amap]
+                    vals.append(np.broadcast_to(np.asarray(fm(*proxies, *([0] * ndim)), dtype=np.float64),
+                                                (TY, TZ)))
                 # the kernel may skip (and zero) cells outside the output tile
                 # widened by (TT-L) radii; zero them all here, so a needed
                 # cell skipped there fails the comparison with the oracle
                 wy, wz = (TT - L) * rad[1], (TT - L) * rad[2]
                 need = (ly >= GY - wy) & (ly < GY + OTY + wy) & (lzv + VZ > GZ - wz) & (lzv < GZ + OTZ + wz)
-                val = np.where(need, val, 0.0)
+                vals = [np.where(need, v, 0.0) for v in vals]
                 if L == TT:
                     if p >= xb:
                         if not (0 <= p < n0):
                             errs.append(f"store plane {p} outside the interior")
                             continue
                         m = mine
-                        res[h0 + p, h1 + y[m], h2 + z[m]] = val[m]
+                        for fi in range(NF):
+                            res[fi][h0 + p, h1 + y[m], h2 + z[m]] = vals[fi][m]
                         written[h0 + p, h1 + y[m], h2 + z[m]] += 1
                 else:
                     pin = w0 or (0 <= p < n0)
                     sl = _pmod(R - (L - 1) * D - QLO, QN)
-                    if tmode == 0:
-                        q[L][sl] = np.where(pin & keep, val, 0.0)
-                    else:
-                        q[L][sl] = np.array(val, copy=True)
+                    for fi in range(NF):
+                        if tmode == 0:
+                            q[L][fi][sl] = np.where(pin & keep, vals[fi], 0.0)
+                        else:
+                            q[L][fi][sl] = np.array(vals[fi], copy=True)
     inner = tuple(slice(h, s - h) for h, s in zip((h0, h1, h2), u3.shape))
     if written.max() > 1 or (not edges and not (written[inner] == 1).all()):
         errs.append(f"interior cells written {written[inner].min()}..{written[inner].max()} times")
@@ -265,4 +292,4 @@ def run(fn, ndim, cfg, plan, qrange, u, halo, xch, tmode=0, wrap=0, out=None, ed
     wmask[inner] = False
     if written[wmask].any():
         errs.append("halo cells written")
-    return res.reshape(np.shape(u)), errs
+    return [r.reshape(np.shape(us[0])) for r in res], errs

@@ -162,3 +162,35 @@ def test_serving_solo_launch_in_bounds(kind, dtype, side, T):
     got, errs = tm.run(fn._fn, len(shape), cfg, plan_accesses(meta), tblock_queue_range(meta), np.zeros(shape),
                        halo, args.xch, tmode=0, wrap=0, edges=True)
     assert not errs, errs[:5]
+
+
+@pytest.mark.parametrize("dtype,T,n,wrap", [("float64", 2, 4096, 5), ("float64", 4, 4096, 5), ("float32", 2, 4096, 5),
+                                            ("float32", 4, 4096, 5), ("float64", 2, 1000, 0), ("float64", 4, 999, 0)])
+def test_fused_block_launch_in_bounds(dtype, T, n, wrap):
+    """Launches of the coupled-field temporal kernel (fused Gray-Scott) that
+    ``FusedStepper`` makes: the benchmark size wrapped, and zero-Dirichlet
+    ones, on the model's edge workgroups."""
+    from cutile import compile_fused
+    from cutile.lowering.meta import extract_fused_meta
+    from cutile.runtime.launcher import compile_fused_block
+
+    fns = list(w.gray_scott(dtype))
+    one = compile_fused(fns)
+    blk = compile_fused_block(fns, T, one)
+    halo = blk.halo_widths
+    shape = tuple(n + 2 * h for h in halo)
+    kern = blk.load_module()._KERNEL
+    dt = getattr(torch, K.STORAGE_TORCH[kern.storage])
+    ins = [torch.zeros(shape, dtype=dt) for _ in range(2)]
+    key = kern.temporal_key(ins, ins, halo)
+    assert key.startswith("tblock"), kern.variants.keys()
+    args, _ = kern.args(ins, ins, halo, key)
+    meta = extract_fused_meta([f._ir for f in fns])
+    vz = kern.variants[key][3]
+    cfg = tblock_config(meta, T, blk.storage, blk.storage, vz)
+    assert tblock_symbol(meta, cfg) == kern.variants[key][0]
+    names = list(meta.input_names)
+    argmap = [tm._arg_fields(f._fn, 2, names) for f in fns]
+    _, errs = tm.run_fields([f._fn for f in fns], argmap, 2, cfg, plan_accesses(meta), tblock_queue_range(meta),
+                            [np.zeros(shape), np.zeros(shape)], halo, args.xch, tmode=0, wrap=wrap, edges=True)
+    assert not errs, errs[:5]

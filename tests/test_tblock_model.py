@@ -110,3 +110,61 @@ def test_geometry_invariants(name, make, T, storage, vz, n, halo):
     assert cfg.gz % cfg.vz == 0 and cfg.tz % cfg.vz == 0 and cfg.otz % cfg.vz == 0
     assert cfg.oty > 0 and cfg.otz > 0
     assert cfg.kc * cfg.nt >= cfg.ty * cfg.tz // cfg.vz > (cfg.kc - 1) * cfg.nt
+
+
+def _ref_zero_multi(fields, fns, names, ndim, halo, T):
+    """T chained fused sweeps of coupled fields; levels >= 1 pass through
+    zero-halo arrays (result m is the next state of field ``names[m]``)."""
+    cur = {k: v.copy() for k, v in fields.items()}
+    shape = next(iter(fields.values())).shape
+    inner = oracle.interior(shape, halo)
+    for t in range(T):
+        outs = oracle.apply_multi(cur, fns, ndim, halo, base=[cur[n] for n in names])
+        nxt = {}
+        for n, o in zip(names, outs):
+            if t + 1 < T:
+                z = np.zeros_like(o)
+                z[inner] = o[inner]
+                nxt[n] = z
+            else:
+                nxt[n] = o
+        cur = nxt
+    return cur
+
+
+@pytest.mark.parametrize("mode", ["zero", "wrap"])
+@pytest.mark.parametrize("T,vz,n,halo", [(2, 2, (37, 300), (1, 16)), (4, 2, (23, 262), (1, 16)),
+                                         (2, 1, (30, 97), (1, 3)), (3, 1, (19, 140), (1, 1))])
+def test_coupled_fields_model_matches_oracle(T, vz, n, halo, mode):
+    """Fused Gray-Scott (two coupled fields) at temporal depth T: the model
+    of the generalised kernel (a queue and LDS planes per field and level)
+    equals T fused steps of the oracle, zero-halo and periodic-wrap."""
+    from cutile.lowering.meta import extract_fused_meta
+
+    fns = list(w.gray_scott("float64"))
+    meta = extract_fused_meta([f._ir for f in fns])
+    names = list(meta.input_names)
+    assert names == ["u", "v"] and meta.num_exprs == 2
+    cfg = tblock_config(meta, T, "float64", "float64", vz)
+    assert cfg is not None
+    plan, qr = plan_accesses(meta), tblock_queue_range(meta)
+    rng = np.random.default_rng(T * 10 + vz)
+    shape = tuple(a + 2 * h for a, h in zip(n, halo))
+    fields = {k: rng.random(shape) for k in names}
+    if mode == "zero":
+        ref = _ref_zero_multi(fields, [f._fn for f in fns], names, 2, halo, T)
+        tmode, wrap = 0, 0
+    else:
+        fp = {k: v.copy() for k, v in fields.items()}
+        for v in fp.values():
+            oracle.apply_boundary(v, w.periodic_spec(2), halo)
+        ref = oracle.time_march_multi(fp, [f._fn for f in fns], names, 2, halo, T, w.periodic_spec(2))
+        tmode, wrap = 0, 5
+    argmap = [tm._arg_fields(f._fn, 2, names) for f in fns]
+    inner = oracle.interior(shape, halo)
+    for xch in (3, 7, n[0]):
+        got, errs = tm.run_fields([f._fn for f in fns], argmap, 2, cfg, plan, qr, [fields[k] for k in names], halo,
+                                  xch, tmode=tmode, wrap=wrap)
+        assert not errs, (xch, errs[:5])
+        for k, g in zip(names, got):
+            np.testing.assert_array_equal(g[inner], ref[k][inner], err_msg=f"{k} xch={xch}")
