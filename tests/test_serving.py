@@ -32,7 +32,7 @@ def _mix():
     spec = [("heat2d", "float32", 70, 12), ("heat2d", "float64", 200, 25), ("diffusion3d", "float32", 20, 9),
             ("diffusion3d", "float64", 33, 17), ("grayscott", "float32", 50, 30), ("grayscott", "float64", 130, 6),
             ("heat2d", "float32", 333, 3), ("grayscott", "float32", 129, 21), ("diffusion3d", "float32", 45, 14),
-            ("diffusion3d", "float32", 230, 5), ("heat2d", "float32", 2050, 6)]
+            ("diffusion3d", "float32", 230, 5), ("heat2d", "float32", 2050, 6), ("grayscott", "float32", 2801, 5)]
     return [Job(i, k, d, s, n, arrival=0.0) for i, (k, d, s, n) in enumerate(spec)]
 
 
@@ -61,12 +61,14 @@ def test_float64_jobs_match_the_oracle(oracle):
     from cutile.serving import SOLO, Job, StencilServer, _Class
 
     print("serving: float64 jobs vs oracle", flush=True)
-    # jobs 3 and 4 are stepped alone: depth-2 launches (two, then one sweep
-    # for the fifth step) and depth-4 launches (one, then two sweeps)
+    # jobs 3-5 are stepped alone: depth-2 launches (two, then one sweep for
+    # the fifth step) and depth-4 launches (one, then two sweeps; job 5
+    # through the coupled-field kernel)
     jobs = [Job(0, "heat2d", "float64", 90, 11), Job(1, "diffusion3d", "float64", 26, 7),
             Job(2, "grayscott", "float64", 61, 9),
             Job(3, "diffusion3d", "float64", SOLO["diffusion3d", "float64"][0], 5),
-            Job(4, "heat2d", "float64", SOLO["heat2d", "float64"][0], 6)]
+            Job(4, "heat2d", "float64", SOLO["heat2d", "float64"][0], 6),
+            Job(5, "grayscott", "float64", SOLO["grayscott", "float64"][0], 6)]
     srv = StencilServer(keep_results=True)
     srv.run([Job(j.id, j.kind, j.dtype, j.side, j.steps) for j in jobs], realtime=False)
     for j in jobs:

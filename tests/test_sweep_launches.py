@@ -120,3 +120,27 @@ def test_3d_sweep_chunks_fill_whole_rounds(n0, tiles, qd, wgs):
     of 77 planes instead of 288 of 64 (a second, nearly empty round)."""
     x = K.sweep_chunk_length(n0, tiles, 256, qd)
     assert tiles * -(-n0 // x) == wgs
+
+
+@pytest.mark.parametrize("dtype,n", [("float64", 1536), ("float64", 1537), ("float64", 4095), ("float32", 2800),
+                                     ("float32", 2801), ("float32", 4095)])
+def test_serving_solo_gray_scott_steps_in_bounds(dtype, n):
+    """The single fused steps (periodic, wrapped reads) that end the serving
+    loop's solo Gray-Scott jobs."""
+    from cutile import workloads as w
+
+    fns = list(w.gray_scott(dtype))
+    r = compile_fused(fns)
+    kern = r.load_module()._KERNEL
+    shape = tuple(n + 2 * h for h in r.halo_widths)
+    dt = getattr(torch, K.STORAGE_TORCH[kern.storage])
+    ins = [torch.zeros(shape, dtype=dt) for _ in range(2)]
+    key = kern.pick(ins, ins, r.halo_widths)
+    a, nblocks = kern.args(ins, ins, r.halo_widths, key)
+    a.wrap = kern.wrap_mask((0, 1), a)
+    _, ty, tz, vz, nt = kern.variants[key]
+    meta = extract_fused_meta([f._ir for f in fns])
+    errs = km.check_launch(KernelConfig(ty, tz, vz, nt), plan_accesses(meta), (a.s0, a.s1, a.s2), (a.h0, a.h1, a.h2),
+                           (a.n0, a.n1, a.n2), a.xch, a.yslab, accesses=access_table(meta), wrap=a.wrap,
+                           elem_bytes=8 if dtype == "float64" else 4)
+    assert not errs, errs[:5]

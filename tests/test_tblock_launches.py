@@ -165,7 +165,10 @@ def test_serving_solo_launch_in_bounds(kind, dtype, side, T):
 
 
 @pytest.mark.parametrize("dtype,T,n,wrap", [("float64", 2, 4096, 5), ("float64", 4, 4096, 5), ("float32", 2, 4096, 5),
-                                            ("float32", 4, 4096, 5), ("float64", 2, 1000, 0), ("float64", 4, 999, 0)])
+                                            ("float32", 4, 4096, 5), ("float64", 2, 1000, 0), ("float64", 4, 999, 0),
+                                            # the serving loop's solo Gray-Scott jobs
+                                            ("float64", 4, 1536, 5), ("float64", 4, 1537, 5), ("float64", 4, 4095, 5),
+                                            ("float32", 4, 2800, 5), ("float32", 4, 2801, 5), ("float32", 4, 4095, 5)])
 def test_fused_block_launch_in_bounds(dtype, T, n, wrap):
     """Launches of the coupled-field temporal kernel (fused Gray-Scott) that
     ``FusedStepper`` makes: the benchmark size wrapped, and zero-Dirichlet
