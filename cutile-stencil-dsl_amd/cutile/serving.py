@@ -330,8 +330,6 @@ def init_fields(job: Job, cls: _Class, bufs: dict) -> None:
     periodic Gray-Scott halo is filled once here."""
     import torch
 
-    from cutile.runtime import boundary
-
     g = torch.Generator(device="cuda").manual_seed(int(job.id))
     rng = np.random.default_rng(int(job.id))
     s, h = job.side, cls.halo_of(job)
