@@ -26,3 +26,45 @@ def test_nonzero_dirichlet_steps_one_at_a_time():
     assert st.T == 1 and st.block is None
     st = TimeStepper(w.heat2d(), w.dirichlet_spec(2), temporal_steps=4)
     assert st.T == 4
+
+
+def test_fused_block_needs_a_closed_system():
+    """``compile_fused_block``: one member per input, member m advancing
+    input m, depth at least 2; ``compile_fused`` itself keeps upstream's
+    one-step restriction."""
+    import pytest
+
+    from cutile import compile_fused
+    from cutile.runtime.launcher import compile_fused_block
+
+    fns = list(w.gray_scott())
+    one = compile_fused(fns)
+    blk = compile_fused_block(fns, 2, one)
+    assert blk.temporal_steps == 2 and blk.inputs == ("u", "v") and blk.halo_widths == one.halo_widths
+    assert "tblock" in blk.load_module()._KERNEL.variants
+    with pytest.raises(ValueError):
+        compile_fused_block(fns[:1], 2, compile_fused(fns[:1]))    # gs_u alone: one result, two inputs
+    with pytest.raises(ValueError):
+        compile_fused_block(fns, 1, one)
+    with pytest.raises(NotImplementedError):
+        compile_fused(fns, temporal_steps=2)
+
+
+def test_fused_stepper_uses_blocks_only_where_they_are_exact():
+    import pytest
+
+    from cutile import compile_fused
+    from cutile.runtime.launcher import compile_fused_block
+    from cutile.runtime.timestep import FusedStepper
+
+    fns = list(w.gray_scott())
+    one = compile_fused(fns)
+    blk = compile_fused_block(fns, 4, one)
+    assert FusedStepper(one, w.periodic_spec(2), ["u", "v"], block=blk).T == 4
+    assert FusedStepper(one, w.dirichlet_spec(2), ["u", "v"], block=blk).T == 4
+    # a refilled (Neumann) halo between steps is not what zero-halo levels compute
+    neumann = {0: ("neumann", "neumann"), 1: ("neumann", "neumann")}
+    st = FusedStepper(one, neumann, ["u", "v"], block=blk)
+    assert st.T == 1 and st._blk is None
+    with pytest.raises(ValueError):
+        FusedStepper(one, w.periodic_spec(2), ["v", "u"], block=blk)   # results must advance inputs in order
