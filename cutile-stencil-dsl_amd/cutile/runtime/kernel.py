@@ -184,6 +184,14 @@ def chunk_length(n0: int, tiles: int, default: int, queue_depth: int, per_cu: in
     return max(1, min(x, n0))
 
 
+def run_prepared(rec, stream=None) -> None:
+    """Launch a packed record of ``StencilKernel.prepare`` on ``stream``."""
+    fn, nblocks, nt, addr, size, _args = rec
+    if stream is None:
+        stream = default_stream()
+    rt.check(rt.lib().st_launch(fn, nblocks, 1, 1, nt, 1, 1, 0, stream, addr, size), "st_launch")
+
+
 def sweep_chunk_length(n0: int, tiles: int, default: int, queue_depth: int) -> int:
     """Planes per workgroup of a 3-D sweep: one long-streaming workgroup
     per CU, the chunk count chosen by the round model of
@@ -407,26 +415,40 @@ class StencilKernel:
                tuple(halo), xch, tuple(periodic_axes), int(mode), "temporal")
         hit = self._launches.get(sig)
         if hit is None:
-            if validate:
-                self.check_arrays(inputs, outputs)
-            key = self.temporal_key(inputs, outputs, halo)
-            args, nblocks = self.args(inputs, outputs, halo, key, xch)
-            if args is None:
+            hit = self.prepare(inputs, outputs, halo, temporal=True, mode=mode, xch=xch,
+                               periodic_axes=periodic_axes, validate=validate)
+            if hit is None:
                 return
+            if len(self._launches) >= 256:
+                self._launches.clear()
+            self._launches[sig] = hit
+        run_prepared(hit, stream)
+
+    def prepare(self, inputs: Sequence, outputs: Sequence, halo: Sequence[int], temporal: bool = False,
+                mode: int = 0, xch: Optional[int] = None, periodic_axes: Sequence[int] = (),
+                validate: bool = True):
+        """The packed launch of one sweep (``temporal``: one block of T
+        steps, as ``launch_temporal``) on these arrays, for
+        ``run_prepared``; None for an empty interior.  A caller that
+        launches the same arrays many times (the serving loop's solo jobs)
+        keeps it rather than relying on the signature cache."""
+        if validate:
+            self.check_arrays(inputs, outputs)
+        key = self.temporal_key(inputs, outputs, halo) if temporal else self.pick(inputs, outputs, halo)
+        args, nblocks = self.args(inputs, outputs, halo, key, xch)
+        if args is None:
+            return None
+        if temporal:
             args.tmode = int(mode)
             if periodic_axes:
                 if not key.startswith("tblock"):
                     raise ValueError(f"launch_{self.name}: periodic reads need the register/LDS temporal kernel")
                 args.wrap = self.wrap_mask(periodic_axes, args, need_halo=False)
-            hit = (self._function(key), nblocks, self.variants[key][4], ctypes.addressof(args), ctypes.sizeof(args),
-                   args)
-            if len(self._launches) >= 256:
-                self._launches.clear()
-            self._launches[sig] = hit
-        fn, nblocks, nt, addr, size, _args = hit
-        if stream is None:
-            stream = default_stream()
-        rt.check(rt.lib().st_launch(fn, nblocks, 1, 1, nt, 1, 1, 0, stream, addr, size), "st_launch")
+        elif periodic_axes:
+            args.wrap = self.wrap_mask(periodic_axes, args)
+        # everything st_launch takes but the stream (args stays referenced)
+        return (self._function(key), nblocks, self.variants[key][4], ctypes.addressof(args), ctypes.sizeof(args),
+                args)
 
     def launch_bricks(self, inputs: Sequence, outputs: Sequence, shape: Sequence[int], halo: Sequence[int],
                       stream=None, xch: Optional[int] = None) -> None:
@@ -508,21 +530,10 @@ class StencilKernel:
                tuple(halo), xch, tuple(periodic_axes))
         hit = self._launches.get(sig)
         if hit is None:
-            if validate:
-                self.check_arrays(inputs, outputs)
-            key = self.pick(inputs, outputs, halo)
-            args, nblocks = self.args(inputs, outputs, halo, key, xch)
-            if args is None:
+            hit = self.prepare(inputs, outputs, halo, xch=xch, periodic_axes=periodic_axes, validate=validate)
+            if hit is None:
                 return
-            if periodic_axes:
-                args.wrap = self.wrap_mask(periodic_axes, args)
-            # everything st_launch takes but the stream (args stays referenced)
-            hit = (self._function(key), nblocks, self.variants[key][4], ctypes.addressof(args), ctypes.sizeof(args),
-                   args)
             if len(self._launches) >= 256:
                 self._launches.clear()
             self._launches[sig] = hit
-        fn, nblocks, nt, addr, size, _args = hit
-        if stream is None:
-            stream = default_stream()
-        rt.check(rt.lib().st_launch(fn, nblocks, 1, 1, nt, 1, 1, 0, stream, addr, size), "st_launch")
+        run_prepared(hit, stream)

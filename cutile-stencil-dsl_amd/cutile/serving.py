@@ -297,6 +297,7 @@ class _Class:
         (one launch for all), solo jobs by T steps (one temporal launch each;
         single sweeps for a job's last steps short of a block)."""
         from cutile.runtime import rt
+        from cutile.runtime.kernel import run_prepared
 
         flip = self.tick % 2
         plan = self.plans[flip] if self.plans else None
@@ -311,14 +312,18 @@ class _Class:
                 j.done += 1
                 continue
             sol = self.solo
-            ins = [j.bufs[f][j.cur] for f in self.fields]
-            outs = [j.bufs[f][1 - j.cur] for f in self.fields]
-            if j.steps - j.done >= sol.T:
-                sol.block.launch_temporal(ins, outs, sol.halo, stream, mode=0, validate=False, periodic_axes=sol.wrap)
-                j.done += sol.T
-            else:
-                sol.single.launch(ins, outs, sol.halo, stream, validate=False, periodic_axes=sol.wrap)
-                j.done += 1
+            block = j.steps - j.done >= sol.T
+            rec = j.recs[j.cur][block] if j.recs else None
+            if rec is None:                       # packed once per buffer parity and kind
+                if not j.recs:
+                    j.recs = [[None, None], [None, None]]
+                ins = [j.bufs[f][j.cur] for f in self.fields]
+                outs = [j.bufs[f][1 - j.cur] for f in self.fields]
+                rec = (sol.block.prepare(ins, outs, sol.halo, temporal=True, mode=0, periodic_axes=sol.wrap)
+                       if block else sol.single.prepare(ins, outs, sol.halo, periodic_axes=sol.wrap))
+                j.recs[j.cur][block] = rec
+            run_prepared(rec, stream)
+            j.done += sol.T if block else 1
             j.cur = 1 - j.cur
 
 
