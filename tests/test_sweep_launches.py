@@ -1,7 +1,7 @@
 """The sweep launches of the application drivers and the benchmark suite,
 replayed on the host bounds model (CPU only).
 
-Each case compiles the stencil as the driver does, builds the launch
+Each case compiles the stencil as the application script does, builds the launch
 arguments through ``StencilKernel.args`` (256 CUs, the MI355X count; the
 variant the launch picks) and checks every global and LDS access of the
 edge workgroups with ``kernel_model.check_launch`` before any of them runs
