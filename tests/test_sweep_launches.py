@@ -144,3 +144,25 @@ def test_serving_solo_gray_scott_steps_in_bounds(dtype, n):
                            (a.n0, a.n1, a.n2), a.xch, a.yslab, accesses=access_table(meta), wrap=a.wrap,
                            elem_bytes=8 if dtype == "float64" else 4)
     assert not errs, errs[:5]
+
+
+def test_prepared_wrapped_heat_sweep_in_bounds():
+    """The wrapped single sweep that ``test_gpu_parity.py`` replays from a
+    prepared record (heat2d FP64, 96^2, periodic reads on both axes); the
+    arguments are built as ``StencilKernel.prepare`` builds them."""
+    from cutile import workloads as w
+
+    fn = w.heat2d()
+    r = compile(fn, temporal_blocking=False)
+    kern = r.load_module()._KERNEL
+    shape = tuple(96 + 2 * h for h in r.halo_widths)
+    u = torch.zeros(shape, dtype=torch.float64)
+    key = kern.pick([u], [u], r.halo_widths)
+    a, nblocks = kern.args([u], [u], r.halo_widths, key)
+    a.wrap = kern.wrap_mask((0, 1), a)
+    _, ty, tz, vz, nt = kern.variants[key]
+    assert nblocks == a.ntz * a.nty * a.nch
+    meta = _single(fn)
+    errs = km.check_launch(KernelConfig(ty, tz, vz, nt), plan_accesses(meta), (a.s0, a.s1, a.s2), (a.h0, a.h1, a.h2),
+                           (a.n0, a.n1, a.n2), a.xch, a.yslab, accesses=access_table(meta), wrap=a.wrap, elem_bytes=8)
+    assert not errs, errs[:5]

@@ -66,6 +66,7 @@ DEVICE_CASES = [
     ("star25", 2, (24, 20, 36), 1),
     ("diffusion7", 4, (30, 18, 50), 1),
     ("heat2d", 4, (100, 100), 0),
+    ("heat2d", 4, (96, 96), 0),                    # the prepared-record case of test_gpu_parity.py
     ("diffusion7", 2, (16, 20, 24), 1),
     ("heat2d", 3, (33, 47), 0),
     ("heat2d", 3, "validate", 0),       # CompileResult.validate: 64 + 2*T*h per axis, halo included
