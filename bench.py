@@ -274,7 +274,7 @@ def gray_scott_c3(n=4096, steps=1000):
     spec = periodic_spec(2)
     r64 = compile_fused(list(gray_scott("float64")))
     r32 = compile_fused(list(gray_scott("float32")))
-    blocks = {(dt, T): compile_fused_block(list(gray_scott(dt)), T, r) for dt, r in (("float64", r64), ("float32", r32))
+    blocks = {(dt, T): compile_fused_block(list(gray_scott(dt)), T, r, ["u", "v"]) for dt, r in (("float64", r64), ("float32", r32))
               for T in (2, 4)}
     variants = [
         ("float64", r64, None, torch.float64),

@@ -163,12 +163,14 @@ class BatchedStencil:
     def _function(self):
         import torch
 
-        from cutile.runtime.kernel import load_or_compile
+        from cutile.runtime.kernel import load_checked
 
         dev = torch.cuda.current_device()
         if dev not in self._fn:
-            mod = load_or_compile(self.source, f"{self.symbol}.hip")
-            self._fn[dev] = rt.get_function(mod, self.symbol)
+            # refused before loading when the code object uses scratch,
+            # more than 256 VGPR+AGPR or more than 160 KiB of LDS
+            _mod, fns = load_checked(self.source, f"{self.symbol}.hip", [self.symbol])
+            self._fn[dev] = fns[self.symbol]
         return self._fn[dev]
 
     def resources(self):

@@ -128,15 +128,122 @@ def precompile(source: str, arch: str = "gfx950", directory: Optional[str] = Non
     return path
 
 
+def code_object(source: str, arch: str, name: str = "stencil.hip") -> bytes:
+    """The code object of ``source`` for ``arch``: the one written ahead of
+    time when there is one, else the on-disk cache, else a hiprtc build
+    (then cached).  Needs no device."""
+    opts = compile_options(arch)
+    key = rt.cache_key(source, opts)
+    path = os.path.join(aot_dir(), key + ".hsaco")
+    if os.path.exists(path):
+        with open(path, "rb") as f:
+            return f.read()
+    d = cache_dir()
+    path = os.path.join(d, key + ".hsaco") if d else ""
+    if path and os.path.exists(path):
+        with open(path, "rb") as f:
+            code = f.read()
+        if code:
+            return code
+    code = rt.compile_only(source, opts, name=name)
+    if path:
+        tmp = f"{path}.tmp{os.getpid()}"
+        try:
+            with open(tmp, "wb") as f:
+                f.write(code)
+            os.replace(tmp, path)
+        except OSError:
+            pass
+    return code
+
+
+_notes: dict = {}
+
+# Limits every launched kernel meets: no scratch (register spills), at most
+# 256 VGPR+AGPR per lane, the 160 KiB LDS of a gfx950 CU.  A tuning run of
+# 7-point tiles whose code objects broke the first two ended in a GPU memory
+# fault; such variants are refused before they can reach a device.
+MAX_REGISTERS = 256
+MAX_LDS_BYTES = 160 * 1024
+
+
+def code_resources(source: str, arch: Optional[str] = None) -> dict:
+    """Kernel symbol -> ``regcheck.Resources`` (VGPR+AGPR, scratch, LDS)
+    read from the notes of ``source``'s code object for ``arch`` (default:
+    the current device's, or gfx950 without a device)."""
+    from cutile.runtime import regcheck
+
+    arch = arch or default_arch()
+    k = (rt.cache_key(source, compile_options(arch)), arch)
+    res = _notes.get(k)
+    if res is None:
+        res = regcheck.code_object_resources(code_object(source, arch))
+        _notes[k] = res
+    return res
+
+
+def default_arch() -> str:
+    torch = _torch()
+    return current_arch() if torch.cuda.is_available() else "gfx950"
+
+
+def assert_launchable(source: str, symbol: str, arch: Optional[str] = None) -> None:
+    """Raise ``RuntimeError`` unless kernel ``symbol`` of ``source`` uses no
+    scratch, at most ``MAX_REGISTERS`` VGPR+AGPR and at most
+    ``MAX_LDS_BYTES`` of LDS (read from its code object, before anything is
+    loaded on a device)."""
+    res = code_resources(source, arch).get(symbol)
+    if res is None:
+        if regcheck_available():
+            raise RuntimeError(f"{symbol}: kernel not found in its code object")
+        return          # no notes reader: the loaded function's attributes are checked instead
+    why = []
+    if res.stack_bytes > 0:
+        why.append(f"{res.stack_bytes} B of scratch per thread (register spills)")
+    if res.registers > MAX_REGISTERS:
+        why.append(f"{res.registers} VGPR+AGPR > {MAX_REGISTERS}")
+    if res.lds_bytes > MAX_LDS_BYTES:
+        why.append(f"{res.lds_bytes} B of LDS > {MAX_LDS_BYTES}")
+    if why:
+        raise RuntimeError(f"{symbol}: {', '.join(why)}; variant refused, not launched "
+                           f"(choose a smaller tile)")
+
+
+def regcheck_available() -> bool:
+    from cutile.runtime import regcheck
+
+    return regcheck.readelf_path() is not None
+
+
+def check_loaded(function, symbol: str) -> None:
+    """The same limits on a loaded function's attributes (scratch, VGPRs)."""
+    res = rt.function_resources(function)
+    if res.local_bytes > 0 or res.num_regs > MAX_REGISTERS:
+        raise RuntimeError(f"{symbol}: {res.local_bytes} B of scratch per thread, {res.num_regs} VGPRs; "
+                           f"variant refused, not launched")
+
+
+def load_checked(source: str, name: str, symbols: Sequence[str]):
+    """Module of ``source`` on the current device, with every kernel in
+    ``symbols`` checked against the launch limits first: from the code
+    object's notes before it is loaded, and from the loaded functions'
+    attributes.  Returns (module, {symbol: function})."""
+    arch = current_arch()
+    for s in symbols:
+        assert_launchable(source, s, arch)
+    mod = rt.load_code(code_object(source, arch, name))
+    fns = {}
+    for s in symbols:
+        fn = rt.get_function(mod, s)
+        check_loaded(fn, s)
+        fns[s] = fn
+    return mod, fns
+
+
 def load_or_compile(source: str, name: str):
     """Module for ``source`` on the current device: a code object written
     ahead of time when there is one, else a JIT build (cached on disk)."""
-    opts = compile_options(current_arch())
-    path = os.path.join(aot_dir(), rt.cache_key(source, opts) + ".hsaco")
-    if os.path.exists(path):
-        with open(path, "rb") as f:
-            return rt.load_code(f.read())
-    return rt.compile_module(source, opts, cache_dir(), name=name)
+    return rt.load_code(code_object(source, current_arch(), name))
 
 
 _arch_cache: dict = {}
@@ -185,7 +292,10 @@ def chunk_length(n0: int, tiles: int, default: int, queue_depth: int, per_cu: in
 
 
 def run_prepared(rec, stream=None) -> None:
-    """Launch a packed record of ``StencilKernel.prepare`` on ``stream``."""
+    """Launch a packed record of ``StencilKernel.prepare`` on ``stream``; a
+    ``None`` record (empty interior) launches nothing, as ``launch`` does."""
+    if rec is None:
+        return
     fn, nblocks, nt, addr, size, _args = rec
     if stream is None:
         stream = default_stream()
@@ -280,23 +390,28 @@ class StencilKernel:
         self._lock = threading.Lock()
 
     # -- compilation ------------------------------------------------------
+    def assert_launchable(self, key: str) -> None:
+        """Refuse variant ``key`` (every kind: vector, scalar, temporal,
+        brick-major) when its code object uses scratch, more than 256
+        VGPR+AGPR or more than 160 KiB of LDS.  Needs no device."""
+        assert_launchable(self.source, self.variants[key][0])
+
     def _function(self, key: str):
         dev = _torch().cuda.current_device()
         k = (dev, key)
         fn = self._fns.get(k)
         if fn is None:
             with self._lock:
+                sym = self.variants[key][0]
+                arch = current_arch()
+                assert_launchable(self.source, sym, arch)
                 mod = self._modules.get(dev)
                 if mod is None:
                     rt.check(rt.lib().st_init(dev), "st_init")
                     mod = load_or_compile(self.source, f"{self.name}.hip")
                     self._modules[dev] = mod
-                fn = rt.get_function(mod, self.variants[key][0])
-                if key.startswith("tblock"):
-                    res = rt.function_resources(fn)
-                    if res.local_bytes > 0:
-                        raise RuntimeError(f"{self.variants[key][0]}: {res.local_bytes} B of scratch per thread "
-                                           f"(register spills); not launched")
+                fn = rt.get_function(mod, sym)
+                check_loaded(fn, sym)
                 self._fns[k] = fn
         return fn
 
@@ -432,9 +547,13 @@ class StencilKernel:
         ``run_prepared``; None for an empty interior.  A caller that
         launches the same arrays many times (the serving loop's solo jobs)
         keeps it rather than relying on the signature cache."""
-        if validate:
+        if len(inputs) != self.num_inputs or len(outputs) != self.num_outputs:
             self.check_arrays(inputs, outputs)
         key = self.temporal_key(inputs, outputs, halo) if temporal else self.pick(inputs, outputs, halo)
+        # the variant's resources are checked before the arrays and the device
+        self.assert_launchable(key)
+        if validate:
+            self.check_arrays(inputs, outputs)
         args, nblocks = self.args(inputs, outputs, halo, key, xch)
         if args is None:
             return None

@@ -469,16 +469,20 @@ def compile_fused(stencil_fns: list, domain=None, hw=None, tile_sizes=None, halo
     )
 
 
-def compile_fused_block(stencil_fns: list, temporal_steps: int, like: CompileResult) -> CompileResult:
+def compile_fused_block(stencil_fns: list, temporal_steps: int, like: CompileResult,
+                        targets: Sequence[str]) -> CompileResult:
     """Temporal-blocking kernel of a fused closed system: member m's result
-    is the next state of input m (fused Gray-Scott: ``gs_u``, ``gs_v`` of
-    ``u``, ``v``), ``temporal_steps`` steps per launch with the halo, tile
-    and storage of the one-step result ``like``.  Used by ``FusedStepper``;
-    ``compile_fused`` keeps the upstream restriction to one step
-    (``launcher.py:652-655``), this goes beyond it.  The launch semantics
-    are the single-field kernel's (``launch_temporal``: zero-halo levels, or
-    periodic wrap), so a block equals ``temporal_steps`` fused steps bit for
-    bit under zero-Dirichlet or periodic boundaries."""
+    is the next state of field ``targets[m]`` (fused Gray-Scott: ``gs_u``,
+    ``gs_v`` advancing ``u``, ``v``), ``temporal_steps`` steps per launch
+    with the halo, tile and storage of the one-step result ``like``.  The
+    kernel feeds result m back as input field m at every intermediate level,
+    so ``targets[m]`` must be the m-th merged input name; any other order
+    raises.  Used by ``FusedStepper``; ``compile_fused`` keeps the upstream
+    restriction to one step (``launcher.py:652-655``), this goes beyond it.
+    The launch semantics are the single-field kernel's (``launch_temporal``:
+    zero-halo levels, or periodic wrap), so a block equals
+    ``temporal_steps`` fused steps bit for bit under zero-Dirichlet or
+    periodic boundaries."""
     from cutile.lowering.meta import extract_fused_meta
     from cutile.lowering.stencil_to_hip import lower_fused_stencils_to_python
 
@@ -486,6 +490,15 @@ def compile_fused_block(stencil_fns: list, temporal_steps: int, like: CompileRes
     meta = extract_fused_meta(modules)
     if list(meta.input_names) != list(like.inputs) or len(meta.output_names) != len(meta.input_names):
         raise ValueError("a fused temporal block needs one member per input, member m advancing input m")
+    targets = list(targets)
+    if len(targets) != len(stencil_fns):
+        raise ValueError(f"a fused temporal block needs one target field per member, got {len(targets)} "
+                         f"for {len(stencil_fns)} members")
+    for m, (t, f) in enumerate(zip(targets, meta.input_names)):
+        if t != f:
+            raise ValueError(f"member {m} ({meta.output_names[m]}) advances field {t!r}, but the block feeds "
+                             f"result {m} back as input field {f!r}: order the members as the inputs "
+                             f"{tuple(meta.input_names)}")
     T = int(temporal_steps)
     if T < 2:
         raise ValueError("temporal_steps must be at least 2")

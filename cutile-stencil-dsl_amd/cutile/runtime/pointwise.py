@@ -57,12 +57,14 @@ class PointwiseKernel:
     def _function(self):
         import torch
 
-        from cutile.runtime.kernel import load_or_compile
+        from cutile.runtime.kernel import load_checked
 
         dev = torch.cuda.current_device()
         if dev not in self._fn:
-            mod = load_or_compile(self.source, "pointwise.hip")
-            self._fn[dev] = rt.get_function(mod, self.symbol)
+            # refused before loading when the code object uses scratch,
+            # more than 256 VGPR+AGPR or more than 160 KiB of LDS
+            _mod, fns = load_checked(self.source, "pointwise.hip", [self.symbol])
+            self._fn[dev] = fns[self.symbol]
         return self._fn[dev]
 
     def __call__(self, outs: Sequence, ins: Sequence, scalars: Sequence[float] = (), stream=None):

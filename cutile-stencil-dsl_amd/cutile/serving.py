@@ -49,6 +49,10 @@ import numpy as np
 KINDS = ("heat2d", "diffusion3d", "grayscott")
 
 
+# marker of a solo job's launch record that has not been packed yet (a packed
+# record is None when the job's interior is empty)
+_UNPREPARED = object()
+
 @dataclass
 class Job:
     id: int
@@ -194,7 +198,7 @@ class _Class:
 
                 # periodic: blocks and single steps read around the interior
                 one = compile_fused(fn)
-                blk = compile_fused_block(fn, depth, one)
+                blk = compile_fused_block(fn, depth, one, ["u", "v"])
                 self.solo = _Solo(one.load_module()._KERNEL, blk.load_module()._KERNEL, tuple(one.halo_widths),
                                   depth, (0, 1))
             else:
@@ -313,15 +317,15 @@ class _Class:
                 continue
             sol = self.solo
             block = j.steps - j.done >= sol.T
-            rec = j.recs[j.cur][block] if j.recs else None
-            if rec is None:                       # packed once per buffer parity and kind
+            rec = j.recs[j.cur][block] if j.recs else _UNPREPARED
+            if rec is _UNPREPARED:                # packed once per buffer parity and kind
                 if not j.recs:
-                    j.recs = [[None, None], [None, None]]
+                    j.recs = [[_UNPREPARED, _UNPREPARED], [_UNPREPARED, _UNPREPARED]]
                 ins = [j.bufs[f][j.cur] for f in self.fields]
                 outs = [j.bufs[f][1 - j.cur] for f in self.fields]
                 rec = (sol.block.prepare(ins, outs, sol.halo, temporal=True, mode=0, periodic_axes=sol.wrap)
                        if block else sol.single.prepare(ins, outs, sol.halo, periodic_axes=sol.wrap))
-                j.recs[j.cur][block] = rec
+                j.recs[j.cur][block] = rec       # None: empty interior, nothing to launch
             run_prepared(rec, stream)
             j.done += sol.T if block else 1
             j.cur = 1 - j.cur

@@ -180,7 +180,7 @@ def test_fused_blocks_match_single_steps(oracle, dtype, T, bc, n):
     print(f"fused blocks: gray-scott {dtype} {n} T={T} {bc}", flush=True)
     fns = list(w.gray_scott(dtype))
     one = compile_fused(fns)
-    blk = compile_fused_block(fns, T, one)
+    blk = compile_fused_block(fns, T, one, ["u", "v"])
     h = one.halo_widths
     spec = w.periodic_spec(2) if bc == "periodic" else w.dirichlet_spec(2)
     shape = tuple(a + 2 * b for a, b in zip(n, h))

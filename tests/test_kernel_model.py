@@ -145,3 +145,46 @@ def test_wrapped_periodic_reads_in_bounds(n):
                 errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=access_table(meta), wrap=7,
                                        elem_bytes=8 if meta.dtype == "float64" else 4)
                 assert not errs, errs[:5]
+
+
+@pytest.mark.parametrize("tile", [(32, 128, 128), (16, 128, 128), (32, 64, 64)])
+@pytest.mark.parametrize("n", [(512, 512, 512), (128, 256, 256), (64, 64, 64), (37, 45, 70)])
+def test_fault_record_tiles_in_bounds(tile, n):
+    """The 7-point tiles of the tuning run that ended in a GPU memory fault
+    (vector and scalar variants, with and without wrapped reads): every
+    global and LDS access of the model stays inside its buffer.  The launch
+    path refuses these variants from their code objects (scratch or more
+    than 256 VGPR+AGPR); this pins that the fault does not follow from the
+    kernel's indexing at these shapes."""
+    meta = extract_meta(find_func(w.diffusion7()._ir))
+    plan = plan_accesses(meta)
+    acc = access_table(meta)
+    vec, scal, xch = make_configs(3, tile, meta.dtype)
+    halo3 = (1, 1, 16)
+    shape3 = tuple(k + 2 * hh for k, hh in zip(n, halo3))
+    for cfg in (vec, scal):
+        if shape3[2] % cfg.vz:
+            continue
+        for x in (xch, 64, 7):
+            for wrap in (0, 7):
+                errs = km.check_launch(cfg, plan, shape3, halo3, n, x, 0, accesses=acc, wrap=wrap)
+                assert not errs, errs[:5]
+
+
+def test_launch_block_matches_launch_bounds():
+    """Every variant is launched with exactly the workgroup size its kernel
+    was compiled for (``__launch_bounds__``), the fault-record tiles and the
+    temporal kernels included."""
+    import re
+
+    from cutile import compile
+
+    results = [compile(w.diffusion7(), tile_sizes=t) for t in ((32, 128, 128), (16, 128, 128), (32, 64, 64))]
+    results += [compile(w.star25()), compile(w.star25(), temporal_steps=2), compile(w.heat2d(), temporal_steps=4),
+                compile(w.diffusion7(), layout="bricked")]
+    for r in results:
+        bounds = dict((s, int(n)) for n, s in
+                      re.findall(r"__launch_bounds__\((\d+)\)\s+(\w+)\(", r.hip_source))
+        kern = r.load_module()._KERNEL
+        for key, (sym, _ty, _tz, _vz, nt) in kern.variants.items():
+            assert bounds[sym] == nt, (key, sym, bounds[sym], nt)

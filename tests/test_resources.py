@@ -91,3 +91,41 @@ def test_sweep_lds_matches_the_host_estimate(fn, ty, tz, vz, nt):
     res = regcheck.code_object_resources(rt.compile_only(src, compile_options("gfx950")))
     (v,) = res.values()
     assert v.lds_bytes == lds_bytes(meta, cfg, compute)
+
+
+@pytest.mark.parametrize("tiles", [(32, 128, 128), (16, 128, 128), (32, 64, 64)])
+def test_oversized_tiles_are_refused_before_launch(tiles):
+    """The 7-point tiles whose tuning run ended in a GPU memory fault spill
+    or exceed 256 VGPR+AGPR in at least one variant; the product launch path
+    refuses such a variant from its code object before it looks at the
+    arrays or a device, and the variants that fit stay launchable."""
+    import torch
+
+    from cutile.runtime.kernel import code_resources
+
+    r = compile(w.diffusion7(), tile_sizes=tiles)
+    kern = r.load_module()._KERNEL
+    res = code_resources(r.hip_source, "gfx950")
+    bad = {k for k, (sym, *_rest) in kern.variants.items()
+           if res[sym].stack_bytes > 0 or res[sym].registers > 256}
+    assert bad, {k: res[v[0]] for k, v in kern.variants.items()}
+    for key in kern.variants:
+        if key in bad:
+            with pytest.raises(RuntimeError, match="refused"):
+                kern.assert_launchable(key)
+        else:
+            kern.assert_launchable(key)
+    if "vec" in bad:
+        # CPU tensors: the refusal comes first, before the device check
+        shape = tuple(16 + 2 * h for h in r.halo_widths)
+        a, b = torch.zeros(shape, dtype=torch.float64), torch.zeros(shape, dtype=torch.float64)
+        with pytest.raises(RuntimeError, match="refused"):
+            r.launcher(a, b)
+
+
+def test_default_tiles_pass_the_launch_guard():
+    from cutile.runtime.kernel import assert_launchable
+
+    for r in (compile(w.diffusion7()), compile(w.star25()), compile(w.star25("float32"))):
+        for sym, *_ in r.load_module()._KERNEL.variants.values():
+            assert_launchable(r.hip_source, sym, "gfx950")

@@ -39,15 +39,33 @@ def test_fused_block_needs_a_closed_system():
 
     fns = list(w.gray_scott())
     one = compile_fused(fns)
-    blk = compile_fused_block(fns, 2, one)
+    blk = compile_fused_block(fns, 2, one, ["u", "v"])
     assert blk.temporal_steps == 2 and blk.inputs == ("u", "v") and blk.halo_widths == one.halo_widths
     assert "tblock" in blk.load_module()._KERNEL.variants
     with pytest.raises(ValueError):
-        compile_fused_block(fns[:1], 2, compile_fused(fns[:1]))    # gs_u alone: one result, two inputs
+        compile_fused_block(fns[:1], 2, compile_fused(fns[:1]), ["u"])    # gs_u alone: one result, two inputs
     with pytest.raises(ValueError):
-        compile_fused_block(fns, 1, one)
+        compile_fused_block(fns, 1, one, ["u", "v"])
     with pytest.raises(NotImplementedError):
         compile_fused(fns, temporal_steps=2)
+
+
+def test_fused_block_rejects_members_out_of_input_order():
+    """Members listed as (gs_v, gs_u) merge to inputs (u, v) but advance
+    (v, u): the block would feed gs_v's result back as field u, so it is
+    refused, and so is a target list whose length does not match."""
+    import pytest
+
+    from cutile import compile_fused
+    from cutile.runtime.launcher import compile_fused_block
+
+    gs_u, gs_v = w.gray_scott()
+    swapped = compile_fused([gs_v, gs_u])
+    assert swapped.inputs == ("u", "v") and swapped.outputs == ("gs_v", "gs_u")
+    with pytest.raises(ValueError, match="advances field"):
+        compile_fused_block([gs_v, gs_u], 2, swapped, ["v", "u"])
+    with pytest.raises(ValueError, match="one target field per member"):
+        compile_fused_block([gs_u, gs_v], 2, compile_fused([gs_u, gs_v]), ["u"])
 
 
 def test_fused_stepper_uses_blocks_only_where_they_are_exact():
@@ -59,7 +77,7 @@ def test_fused_stepper_uses_blocks_only_where_they_are_exact():
 
     fns = list(w.gray_scott())
     one = compile_fused(fns)
-    blk = compile_fused_block(fns, 4, one)
+    blk = compile_fused_block(fns, 4, one, ["u", "v"])
     assert FusedStepper(one, w.periodic_spec(2), ["u", "v"], block=blk).T == 4
     assert FusedStepper(one, w.dirichlet_spec(2), ["u", "v"], block=blk).T == 4
     # a refilled (Neumann) halo between steps is not what zero-halo levels compute
