@@ -2,19 +2,26 @@
 """Benchmark of the stencil hot path on one MI355X.
 
 Headline workload: 3-D 25-point star (8th-order Laplacian, radius 4)
-diffusion step ``u += 0.05*L8(u)`` in float64 on a 512^3 interior with a
-periodic halo, one step = one sweep kernel + one on-device halo fill,
-temporal depth 1.  ``value`` is interior cell-updates per second over the
-timed steps (inputs resident in HBM).  The same JSON line carries:
+diffusion step ``u += 0.05*L8(u)`` in float64 on a 512^3 interior with
+periodic boundaries, temporal depth 1: one step = one sweep kernel whose
+reads wrap around the periodic axes (no halo fill between steps).
+``value`` is interior cell-updates per second over the timed steps
+(inputs resident in HBM): the K steps are captured as one hipGraph and
+one replay is timed between one pair of HIP events.  The same JSON line
+carries:
 
 * ``roofline``: the sweep kernel's algorithmic HBM bytes per launch
   (16 B per interior cell: one float64 read, one write) divided by its
-  average duration (HIP events around every sweep inside the timed loop),
-  against the 8 TB/s MI355X peak;
+  average duration (the timed replay / K, one kernel per step), against
+  the 8 TB/s MI355X peak; ``traffic`` is the kernel's measured HBM bytes
+  per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated, from
+  profiles/traffic.txt);
 * ``cpu_baseline``: the NumPy oracle (single core) on a 256^3 sample of
   the same workload;
-* ``extra``: the 3-D 7-point float64 kernel at 512^3 (target: >= 80 % of
-  the HBM roofline), the float32 25-point kernel, a device copy ceiling.
+* ``extra``: the 3-D 7-point float64 kernel at 512^3, the float32
+  25-point kernel, copy ceilings (three copy forms), temporal depths 1, 2,
+  4 of the 25-point and 7-point stars (50 steps), 2-D heat, Gray-Scott,
+  the batched wave and the 60-s serving stream.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -56,34 +63,18 @@ def traffic_from_profiles(kernel: str):
     return None
 
 
-def run_steps(launch, a, b, steps, stream, rt, record_kernel=True):
-    """Run ``steps`` steps (sweep a->b, swap); returns (total_ms, mean sweep
-    ms, a, b).  Each step is one sweep kernel: the periodic boundary is
-    handled inside it (reads wrap around the interior), so no halo fill runs
-    between steps."""
-    starts = [rt.Event() for _ in range(steps)] if record_kernel else []
-    ends = [rt.Event() for _ in range(steps)] if record_kernel else []
-    e0, e1 = rt.Event(), rt.Event()
-    e0.record(stream)
-    for k in range(steps):
-        if record_kernel:
-            starts[k].record(stream)
-        launch(a, b)
-        if record_kernel:
-            ends[k].record(stream)
-        a, b = b, a
-    e1.record(stream)
-    total = e0.elapsed_ms(e1)
-    kern = (sum(s.elapsed_ms(e) for s, e in zip(starts, ends)) / steps) if record_kernel else None
-    return total, kern, a, b
-
-
-def measure(name, stencil_fn, interior_dev, n, steps, warmup, dtype_name, spec, **compile_kw):
+def measure(name, stencil_fn, interior_dev, n, steps, warmup, dtype_name, spec, warmup_s=0.5, **compile_kw):
+    """``steps`` periodic steps of one stencil, each one wrapped sweep (reads
+    past the interior wrap around, so no halo fill runs between steps),
+    captured once as a hipGraph on a side stream and replayed.  Warm-up:
+    ``warmup`` eager steps, then graph replays until ``warmup_s`` seconds
+    have passed.  The timed region is one replay of the ``steps``-step
+    graph between one pair of HIP events on the launch stream; every step
+    is one kernel, so ``kernel_ms`` = that time / ``steps``."""
     import torch
 
     from cutile import compile
     from cutile.runtime import boundary, rt
-    from cutile.runtime.kernel import default_stream
 
     r = compile(stencil_fn, temporal_blocking=False, **compile_kw)
     halo = r.halo_widths
@@ -94,16 +85,33 @@ def measure(name, stencil_fn, interior_dev, n, steps, warmup, dtype_name, spec, 
     b = torch.zeros_like(a)
     boundary.fill(a, spec, halo)
     K = r.load_module()._KERNEL
-    s = default_stream()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    s = side.cuda_stream
 
-    def launch(x, y):
+    def enqueue(k):
+        x, y = (a, b) if k % 2 == 0 else (b, a)
         K.launch([x], [y], halo, s, validate=False, periodic_axes=(0, 1, 2))
 
-    log(f"{name}: compile + warmup ({warmup} steps), halo {halo}, tile {r.tile_sizes}")
-    _, _, a, b = run_steps(launch, a, b, max(1, warmup), s, rt, record_kernel=False)
-    torch.cuda.synchronize()
-    total_ms, kern_ms, a, b = run_steps(launch, a, b, steps, s, rt)
-    torch.cuda.synchronize()
+    log(f"{name}: compile + warmup ({warmup} steps, then {warmup_s:.1f} s of graph replays), halo {halo}, "
+        f"tile {r.tile_sizes}")
+    with torch.cuda.stream(side):
+        for k in range(max(2, warmup + warmup % 2)):       # an even count: the state is back in a
+            enqueue(k)
+        g = rt.Graph(s)
+        with g:
+            for k in range(steps):
+                enqueue(k)
+        t0 = time.perf_counter()
+        reps = 0
+        while reps < 1 or time.perf_counter() - t0 < warmup_s:
+            g.launch(s)
+            rt.stream_sync(s)
+            reps += 1
+        torch.cuda.synchronize()
+        total_ms = _timed(s, lambda: g.launch(s), rt)
+        torch.cuda.synchronize()
+    kern_ms = total_ms / steps
     cells = n ** 3
     esz = a.element_size()
     key = K.pick([a], [b], halo)
@@ -112,22 +120,28 @@ def measure(name, stencil_fn, interior_dev, n, steps, warmup, dtype_name, spec, 
         "kernel": K.variants[key][0],
         "workload": name, "dtype": dtype_name, "interior": [n, n, n], "halo": list(halo),
         "tile": list(r.tile_sizes), "steps": steps, "ms_per_step": total_ms / steps,
-        "kernel_ms": kern_ms, "gcell_updates_per_s": cells * steps / (total_ms * 1e-3) / 1e9,
-        "kernel_gcells_per_s": cells / (kern_ms * 1e-3) / 1e9,
+        "kernel_ms": kern_ms, "timing": f"one hipGraph of {steps} sweeps, one event pair, "
+                                        f"after {reps} warm-up replays",
+        "gcell_updates_per_s": cells * steps / (total_ms * 1e-3) / 1e9,
         "alg_bytes_per_launch": cells * 2 * esz,
         "kernel_gbs": cells * 2 * esz / (kern_ms * 1e-3) / 1e9,
         "vgprs": res.num_regs, "scratch_bytes": res.local_bytes, "lds_bytes": res.shared_bytes,
     }
     out["hbm_frac"] = out["kernel_gbs"] / PEAK_HBM_GBS
-    log(f"{name}: {out['ms_per_step']:.4f} ms/step, kernel {kern_ms:.4f} ms "
-        f"= {out['kernel_gbs']:.0f} GB/s ({100 * out['hbm_frac']:.1f}% of 8 TB/s), "
-        f"{out['gcell_updates_per_s']:.1f} Gcell-updates/s")
-    del a, b
+    log(f"{name}: {out['ms_per_step']:.4f} ms/step = {out['kernel_gbs']:.0f} GB/s algorithmic "
+        f"({100 * out['hbm_frac']:.1f}% of 8 TB/s), {out['gcell_updates_per_s']:.1f} Gcell-updates/s")
+    del a, b, g
     torch.cuda.empty_cache()
     return out
 
 
 def copy_ceiling(nbytes, iters=20):
+    """Device-to-device copy bandwidth of ``nbytes`` three ways: the
+    runtime's one-vector-per-thread kernel (``st_copy``), its multi-vector
+    form (``st_copy_wide``, 4 and 8 vectors per thread) and the HIP
+    runtime's own ``hipMemcpyAsync`` (called directly in libamdhip64)."""
+    import ctypes
+
     import torch
 
     from cutile.runtime import rt
@@ -136,16 +150,29 @@ def copy_ceiling(nbytes, iters=20):
     src = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
     dst = torch.empty_like(src)
     s = default_stream()
-    for _ in range(3):
-        rt.copy(dst.data_ptr(), src.data_ptr(), nbytes, s)
-    e0, e1 = rt.Event(), rt.Event()
-    e0.record(s)
-    for _ in range(iters):
-        rt.copy(dst.data_ptr(), src.data_ptr(), nbytes, s)
-    e1.record(s)
-    ms = e0.elapsed_ms(e1) / iters
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    hip.hipMemcpyAsync.restype = ctypes.c_int
+    d, p = dst.data_ptr(), src.data_ptr()
+
+    def memcpy():
+        rc = hip.hipMemcpyAsync(d, p, nbytes, 3, s)      # 3 = hipMemcpyDeviceToDevice
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpyAsync failed ({rc})")
+
+    kinds = {"k_copy16": lambda: rt.copy(d, p, nbytes, s),
+             "k_copy16_wide_x4": lambda: rt.copy_wide(d, p, nbytes, 4, s),
+             "k_copy16_wide_x8": lambda: rt.copy_wide(d, p, nbytes, 8, s),
+             "hipMemcpyAsync": memcpy}
+    out = {"bytes_moved": 2 * nbytes}
+    for name, fn in kinds.items():
+        for _ in range(3):
+            fn()
+        ms = _timed(s, lambda: [fn() for _ in range(iters)], rt) / iters
+        out[name] = {"ms": ms, "gbs": 2 * nbytes / (ms * 1e-3) / 1e9}
+    out["gbs"] = max(v["gbs"] for k, v in out.items() if isinstance(v, dict))
     del src, dst
-    return {"bytes_moved": 2 * nbytes, "ms": ms, "gbs": 2 * nbytes / (ms * 1e-3) / 1e9}
+    return out
 
 
 def _timed(stream, fn, rt):
@@ -156,7 +183,7 @@ def _timed(stream, fn, rt):
     return e0.elapsed_ms(e1)
 
 
-def temporal_depths(interior_dev, n, steps=48):
+def temporal_depths(interior_dev, n, steps=50):
     """25-point star (the C2 workload) and 7-point diffusion, float64 and
     float32, temporal depth 1, 2 and 4, periodic: ``steps`` steps as one
     captured hipGraph.  At depth 1 every step is one wrapped sweep; deeper
@@ -333,10 +360,13 @@ def gray_scott_c3(n=4096, steps=1000):
         timed = steps - 2
         if not name.startswith("float64_from"):
             esz = f["u"].element_size()
-            nbytes = (2 + 2 + (4 if upd else 0)) * esz
+            depth = int(name.split("_T")[1]) if "_T" in name else 1
+            # two fields read and written (plus two compensations for the
+            # Kahan form) once per launch, i.e. once per ``depth`` steps
+            nbytes = (2 + 2 + (4 if upd else 0)) * esz / depth
             out[name] = {"ms_per_step": ms / timed, "gcell_updates_per_s": n * n * timed / (ms * 1e-3) / 1e9,
-                         "alg_bytes_per_update": nbytes, "steps": steps,
-                         "kernel_gbs": n * n * nbytes * timed / (ms * 1e-3) / 1e9}
+                         "alg_bytes_per_update": nbytes, "steps": steps, "steps_per_launch": depth,
+                         "alg_gbs": n * n * nbytes * timed / (ms * 1e-3) / 1e9}
             log(f"gray-scott {name}: {ms / timed * 1e3:.1f} us/step, {out[name]['gcell_updates_per_s']:.1f} "
                 f"Gcell-updates/s")
         del f, scratch, g198, g200, stp
@@ -482,7 +512,7 @@ def batched_wave_c4(count=512, steps=200, seed=0):
     return out
 
 
-def serving_c5(seconds=10.0, rate=200.0, seed=0):
+def serving_c5(seconds=60.0, rate=200.0, seed=0):
     """The serving stream (Poisson 200 jobs/s, mixed heat / 3-D diffusion /
     Gray-Scott, 80 % float32) for ``seconds`` of arrivals: p50/p99 latency
     and aggregate Gcell-updates/s; every class's kernels are built before
@@ -583,7 +613,8 @@ def main():
         extra["star25_fp32"] = measure("star25_fp32", star25("float32"), interior, n, args.steps,
                                        args.warmup, "f32", spec)
         extra["copy_ceiling"] = copy_ceiling(head["alg_bytes_per_launch"] // 2)
-        log(f"copy ceiling: {extra['copy_ceiling']['gbs']:.0f} GB/s")
+        log("copy ceiling: " + ", ".join(f"{k} {v['gbs']:.0f} GB/s" for k, v in extra["copy_ceiling"].items()
+                                         if isinstance(v, dict)))
         for key, fn in (() if args.core else (("temporal_depths_c2", lambda: temporal_depths(interior, n)),)):
             try:
                 extra[key] = fn()

@@ -181,6 +181,26 @@ __global__ void __launch_bounds__(256) k_copy16(u32x4* dst, const u32x4* src, si
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
     __builtin_nontemporal_store(src[i], dst + i);
 }
+// V 16-byte vectors per thread, all loads issued before the stores: a copy
+// with V-fold more reads in flight per wave than k_copy16 (the ceiling is
+// reported for both shapes and for hipMemcpyAsync).  Vector k of a
+// workgroup's block is k*256 + tid, so every load instruction of a wave is
+// one contiguous 1 KiB span.
+template <int V>
+__global__ void __launch_bounds__(256) k_copy16_wide(u32x4* dst, const u32x4* src, size_t n16) {
+  const size_t base = (size_t)blockIdx.x * 256 * V + threadIdx.x;
+  u32x4 r[V];
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    const size_t i = base + (size_t)k * 256;
+    if (i < n16) r[k] = __builtin_nontemporal_load(src + i);
+  }
+#pragma unroll
+  for (int k = 0; k < V; ++k) {
+    const size_t i = base + (size_t)k * 256;
+    if (i < n16) __builtin_nontemporal_store(r[k], dst + i);
+  }
+}
 __global__ void k_copy1(unsigned char* dst, const unsigned char* src, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -526,6 +546,28 @@ int st_copy(void* dst, const void* src, size_t nbytes, void* stream) {
   if (done < nbytes) {
     hipLaunchKernelGGL(k_copy1, dim3(grid_for((int64_t)(nbytes - done))), dim3(256), 0, s,
                        (unsigned char*)dst + done, (const unsigned char*)src + done, nbytes - done);
+  }
+  HIPCHK(hipGetLastError());
+  return ST_OK;
+}
+
+int st_copy_wide(void* dst, const void* src, size_t nbytes, int vectors, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if ((((uintptr_t)dst | (uintptr_t)src) & 15) != 0 || nbytes % 16 != 0)
+    return fail(ST_E_INVALID, "st_copy_wide: pointers and size must be multiples of 16 bytes");
+  if (vectors != 1 && vectors != 2 && vectors != 4 && vectors != 8)
+    return fail(ST_E_INVALID, "st_copy_wide: vectors must be 1, 2, 4 or 8 (got %d)", vectors);
+  const size_t n16 = nbytes / 16;
+  if (n16 == 0) return ST_OK;
+  const size_t per = (size_t)256 * vectors;
+  const size_t blocks = (n16 + per - 1) / per;
+  if (blocks > ((size_t)1 << 31) - 1) return fail(ST_E_INVALID, "st_copy_wide: buffer too large");
+  const dim3 g((unsigned)blocks), b(256);
+  switch (vectors) {
+    case 1: hipLaunchKernelGGL(k_copy16_wide<1>, g, b, 0, s, (u32x4*)dst, (const u32x4*)src, n16); break;
+    case 2: hipLaunchKernelGGL(k_copy16_wide<2>, g, b, 0, s, (u32x4*)dst, (const u32x4*)src, n16); break;
+    case 4: hipLaunchKernelGGL(k_copy16_wide<4>, g, b, 0, s, (u32x4*)dst, (const u32x4*)src, n16); break;
+    default: hipLaunchKernelGGL(k_copy16_wide<8>, g, b, 0, s, (u32x4*)dst, (const u32x4*)src, n16); break;
   }
   HIPCHK(hipGetLastError());
   return ST_OK;

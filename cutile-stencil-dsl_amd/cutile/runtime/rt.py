@@ -24,7 +24,7 @@ BC_CODES = {None: 0, "none": 0, "dirichlet": 1, "neumann": 2, "periodic": 3, "re
 EXPORTS = (
     "st_last_error", "st_init", "st_device_count", "st_device_info_get", "st_mem_info",
     "st_compile_only", "st_free", "st_compile", "st_load_code", "st_unload", "st_get_function",
-    "st_launch", "st_function_resources", "st_fill_halo", "st_fill_halo_batched", "st_copy", "st_diff_norms",
+    "st_launch", "st_function_resources", "st_fill_halo", "st_fill_halo_batched", "st_copy", "st_copy_wide", "st_diff_norms",
     "st_bricks",
     "st_event_create", "st_event_record", "st_event_elapsed_ms", "st_event_destroy", "st_event_sync",
     "st_stream_sync", "st_device_sync", "st_graph_begin", "st_graph_end", "st_graph_launch",
@@ -86,6 +86,7 @@ def _declare(lib):
                           ctypes.POINTER(ctypes.c_double), vp], ctypes.c_int),
         "st_fill_halo_batched": ([vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, vp], ctypes.c_int),
         "st_copy": ([vp, vp, ctypes.c_size_t, vp], ctypes.c_int),
+        "st_copy_wide": ([vp, vp, ctypes.c_size_t, ctypes.c_int, vp], ctypes.c_int),
         "st_diff_norms": ([vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, i64p, i64p, vp,
                            ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "st_bricks": ([vp, vp, ctypes.c_int, ctypes.c_int, i64p, ctypes.c_int, ctypes.c_int, vp],
@@ -262,6 +263,10 @@ def fill_halo_batched(descs_dev_ptr: int, count: int, ndim: int, max_slab: int, 
 
 def copy(dst: int, src: int, nbytes: int, stream=None) -> None:
     check(lib().st_copy(dst, src, nbytes, stream), "st_copy")
+
+
+def copy_wide(dst: int, src: int, nbytes: int, vectors: int, stream=None) -> None:
+    check(lib().st_copy_wide(dst, src, nbytes, vectors, stream), "st_copy_wide")
 
 
 def diff_norms(a: int, da: int, b: int, db: int, shape, halo, stream=None) -> tuple:

@@ -17,7 +17,7 @@
  *     (cutile/runtime/regcheck.py:136-212)
  *   cp.cuda.Event timing in benchmark()/autotune()    -> st_event_*
  *     (runtime/launcher.py:208-216, runtime/autotune.py:187-194)
- *   STREAM-copy peak in run_benchmarks.py:38-59       -> st_copy
+ *   STREAM-copy peak in run_benchmarks.py:38-59       -> st_copy, st_copy_wide
  *   to_bricks / from_bricks (multigpu_emitter.py:370-434) -> st_bricks
  *   max-norm comparisons in validate()               -> st_diff_norms
  *     (runtime/launcher.py:127-131)
@@ -140,6 +140,12 @@ int st_fill_halo_batched(const void* descs, int count, int ndim, int64_t max_sla
 
 /* dst[i] = src[i] for nbytes bytes (16-byte vector copy kernel). */
 int st_copy(void* dst, const void* src, size_t nbytes, void* stream);
+
+/* The same copy with `vectors` (1, 2, 4 or 8) 16-byte vectors per thread,
+ * every load issued before the stores, non-temporal loads and stores: a
+ * second copy-bandwidth reference beside st_copy and hipMemcpyAsync.
+ * Pointers and nbytes must be multiples of 16. */
+int st_copy_wide(void* dst, const void* src, size_t nbytes, int vectors, void* stream);
 
 /* Max-norms over the interior of two arrays of the same shape:
  * out[0] = max|a-b|, out[1] = max|b| (computed in FP64). `out` is host memory;
