@@ -220,42 +220,198 @@ struct Shape4 {
   int64_t n[4];
 };
 
-// out[0] = max |a-b| (as ordered uint64 bits), out[1] = max |b|, out[2] = NaN seen
-__global__ void k_diff_norms(const void* a, int da, const void* b, int db, Shape4 sh,
-                             unsigned long long* out) {
-  const int64_t total = sh.n[0] * sh.n[1] * sh.n[2] * sh.n[3];
-  double md = 0.0, mb = 0.0;
-  unsigned nan = 0;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    int64_t r = e, idx = 0;
-    int64_t c[4];
-    for (int d = 3; d >= 0; --d) {
-      c[d] = r % sh.n[d];
-      r /= sh.n[d];
-    }
-    for (int d = 0; d < 4; ++d) idx = idx * sh.s[d] + (c[d] + sh.h[d]);
-    const double x = load_as_f64(a, da, idx), y = load_as_f64(b, db, idx);
-    const double df = fabs(x - y), ab = fabs(y);
-    if (df != df || ab != ab) nan = 1;
-    md = df > md ? df : md;
-    mb = ab > mb ? ab : mb;
+// ---------------------------------------------------------------------------
+// interior reductions (residual norms, field digest): two passes, no atomics
+// ---------------------------------------------------------------------------
+//
+// The interior (padded to 4-D) is walked as rows of n[3] cells.  Partial p
+// covers rows [p*rpp, (p+1)*rpp) with rpp = max(1, RED_CELLS / n[3]); pass 1
+// reduces one partial per workgroup into parts[p], pass 2 reduces the
+// partials in one workgroup.  The order of every reduction is a function of
+// the shape alone, so the FP64 sums are reproducible bit for bit.
+constexpr int64_t RED_CELLS = 65536;
+constexpr int RED_THREADS = 256;
+
+struct Interior {
+  int64_t s[4], h[4], n[4];
+  __host__ __device__ int64_t rows() const { return n[0] * n[1] * n[2]; }
+  __device__ int64_t row_base(int64_t r) const {
+    const int64_t c2 = r % n[2], r1 = r / n[2];
+    const int64_t c1 = r1 % n[1], c0 = r1 / n[1];
+    return (((c0 + h[0]) * s[1] + (c1 + h[1])) * s[2] + (c2 + h[2])) * s[3] + h[3];
   }
-  // wave reduction then one atomic per wave
-  for (int off = 32; off > 0; off >>= 1) {
-    const double od = __shfl_down(md, off), ob = __shfl_down(mb, off);
-    const unsigned on = __shfl_down(nan, off);
-    md = od > md ? od : md;
-    mb = ob > mb ? ob : mb;
-    nan |= on;
+};
+
+int64_t red_rows_per_part(const Interior& in) {
+  const int64_t rpp = in.n[3] > 0 ? RED_CELLS / in.n[3] : 1;
+  return rpp > 0 ? rpp : 1;
+}
+int64_t red_parts(const Interior& in) {
+  const int64_t rows = in.rows();
+  if (rows <= 0 || in.n[3] <= 0) return 0;
+  const int64_t rpp = red_rows_per_part(in);
+  return (rows + rpp - 1) / rpp;
+}
+
+// Norms of a - b: max|a-b|, max|b|, sum (a-b)^2, sum b^2 (FP64), NaN seen.
+struct NormAcc {
+  double v[5];
+};
+struct NormOp {
+  typedef NormAcc Acc;
+  const void* a;
+  const void* b;
+  int da, db;
+  static __device__ Acc zero() { return Acc{{0.0, 0.0, 0.0, 0.0, 0.0}}; }
+  __device__ void cell(Acc& acc, int64_t off, int64_t) const {
+    const double x = load_as_f64(a, da, off), y = load_as_f64(b, db, off);
+    const double d = x - y;
+    const double ad = fabs(d), ay = fabs(y);
+    if (ad != ad || ay != ay) acc.v[4] = 1.0;
+    acc.v[0] = ad > acc.v[0] ? ad : acc.v[0];
+    acc.v[1] = ay > acc.v[1] ? ay : acc.v[1];
+    acc.v[2] += d * d;
+    acc.v[3] += y * y;
   }
-  if ((threadIdx.x & 63) == 0) {
-    const unsigned long long bd = __builtin_bit_cast(unsigned long long, md);
-    const unsigned long long bb = __builtin_bit_cast(unsigned long long, mb);
-    atomicMax(&out[0], bd);
-    atomicMax(&out[1], bb);
-    if (nan) atomicOr(&out[2], 1ull);
+  static __device__ Acc join(const Acc& p, const Acc& q) {
+    Acc r;
+    r.v[0] = q.v[0] > p.v[0] ? q.v[0] : p.v[0];
+    r.v[1] = q.v[1] > p.v[1] ? q.v[1] : p.v[1];
+    r.v[2] = p.v[2] + q.v[2];
+    r.v[3] = p.v[3] + q.v[3];
+    r.v[4] = (p.v[4] != 0.0 || q.v[4] != 0.0) ? 1.0 : 0.0;
+    return r;
   }
+  static __device__ Acc shfl_down(const Acc& p, int off) {
+    Acc r;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) r.v[k] = __shfl_down(p.v[k], off);
+    return r;
+  }
+};
+
+// Position-dependent checksum of the stored bits: the sum, modulo 2^64, of
+// mix64(bits(x_e) ^ e * 0x9e3779b97f4a7c15) over interior cells e (row-major
+// interior index).  mix64 is the finaliser of Steele, Lea and Flood's
+// SplitMix64 ("Fast splittable pseudorandom number generators", OOPSLA 2014).
+__device__ inline uint64_t mix64(uint64_t z) {
+  z ^= z >> 30;
+  z *= 0xbf58476d1ce4e5b9ull;
+  z ^= z >> 27;
+  z *= 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return z;
+}
+struct DigestOp {
+  typedef unsigned long long Acc;
+  const void* a;
+  int da;
+  static __device__ Acc zero() { return 0ull; }
+  __device__ void cell(Acc& acc, int64_t off, int64_t e) const {
+    uint64_t bits;
+    if (da == ST_F64) bits = static_cast<const uint64_t*>(a)[off];
+    else if (da == ST_F32) bits = static_cast<const uint32_t*>(a)[off];
+    else bits = static_cast<const uint16_t*>(a)[off];
+    acc += mix64(bits ^ ((uint64_t)e * 0x9e3779b97f4a7c15ull));
+  }
+  static __device__ Acc join(Acc p, Acc q) { return p + q; }
+  static __device__ Acc shfl_down(Acc p, int off) { return __shfl_down(p, off); }
+};
+
+template <class Op>
+__device__ typename Op::Acc block_reduce(typename Op::Acc acc) {
+  __shared__ typename Op::Acc part[RED_THREADS / 64];
+  for (int off = 32; off > 0; off >>= 1) acc = Op::join(acc, Op::shfl_down(acc, off));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  acc = part[0];
+  for (int w = 1; w < RED_THREADS / 64; ++w) acc = Op::join(acc, part[w]);
+  return acc;
+}
+
+template <class Op>
+__global__ void __launch_bounds__(RED_THREADS) k_reduce_parts(Op op, Interior in, int64_t rpp,
+                                                              typename Op::Acc* parts) {
+  const int64_t p = blockIdx.x;
+  const int64_t r0 = p * rpp;
+  int64_t r1 = r0 + rpp;
+  if (r1 > in.rows()) r1 = in.rows();
+  typename Op::Acc acc = Op::zero();
+  for (int64_t r = r0; r < r1; ++r) {
+    const int64_t base = in.row_base(r);
+    for (int64_t c = threadIdx.x; c < in.n[3]; c += RED_THREADS) op.cell(acc, base + c, r * in.n[3] + c);
+  }
+  acc = block_reduce<Op>(acc);
+  if (threadIdx.x == 0) parts[p] = acc;
+}
+
+template <class Op>
+__global__ void __launch_bounds__(RED_THREADS) k_reduce_final(const typename Op::Acc* parts, int64_t nparts,
+                                                              typename Op::Acc* out) {
+  typename Op::Acc acc = Op::zero();
+  for (int64_t i = threadIdx.x; i < nparts; i += RED_THREADS) acc = Op::join(acc, parts[i]);
+  acc = block_reduce<Op>(acc);
+  if (threadIdx.x == 0) out[0] = acc;
+}
+
+// Enqueue both passes with the partials at `parts` and the result at `out`
+// (device memory), then copy the result to `host` and wait for the stream.
+template <class Op>
+int run_reduction(const Op& op, const Interior& in, typename Op::Acc* parts, typename Op::Acc* out,
+                  typename Op::Acc* host, hipStream_t s) {
+  const int64_t np = red_parts(in);
+  if (np == 0) {
+    *host = typename Op::Acc{};
+    return ST_OK;
+  }
+  if (np > 0x7fffffff) return fail(ST_E_INVALID, "interior too large for one reduction");
+  hipLaunchKernelGGL(k_reduce_parts<Op>, dim3((unsigned)np), dim3(RED_THREADS), 0, s, op, in,
+                     red_rows_per_part(in), parts);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_final<Op>, dim3(1), dim3(RED_THREADS), 0, s, (const typename Op::Acc*)parts, np, out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(host, out, sizeof(*host), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return ST_OK;
+}
+
+int make_interior(int ndim, const int64_t* shape, const int64_t* halo, Interior& in) {
+  if (ndim < 1 || ndim > 4 || !shape || !halo) return fail(ST_E_INVALID, "bad interior description (ndim %d)", ndim);
+  for (int d = 0; d < 4; ++d) {
+    const int k = d - (4 - ndim);
+    in.s[d] = k >= 0 ? shape[k] : 1;
+    in.h[d] = k >= 0 ? halo[k] : 0;
+    in.n[d] = in.s[d] - 2 * in.h[d];
+    if (in.h[d] < 0 || in.n[d] < 0) return fail(ST_E_INVALID, "dim %d: negative halo or interior", k);
+  }
+  return ST_OK;
+}
+
+// Library-held reduction workspace: one buffer per device, grown on demand
+// (hipMalloc; kept until the process ends).  Calls on one device from
+// several threads take the lock for the whole enqueue-and-wait.
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mu;
+std::map<int, Workspace> g_ws;
+
+int workspace(size_t bytes, void** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  Workspace& w = g_ws[dev];
+  if (w.bytes < bytes) {
+    if (w.ptr) HIPCHK(hipFree(w.ptr));
+    w.ptr = nullptr;
+    w.bytes = 0;
+    size_t want = 4096;
+    while (want < bytes) want *= 2;
+    HIPCHK(hipMalloc(&w.ptr, want));
+    w.bytes = want;
+  }
+  *out = w.ptr;
+  return ST_OK;
 }
 
 template <class E>
@@ -573,39 +729,55 @@ int st_copy_wide(void* dst, const void* src, size_t nbytes, int vectors, void* s
   return ST_OK;
 }
 
+static int norms_into(const void* a, int dtype_a, const void* b, int dtype_b, int ndim, const int64_t* shape,
+                      const int64_t* halo, void* stream, double* out, int nout) {
+  if (!a || !b || !out) return fail(ST_E_INVALID, "null argument");
+  Interior in;
+  int rc = make_interior(ndim, shape, halo, in);
+  if (rc) return rc;
+  const int64_t np = red_parts(in);
+  NormAcc host{};
+  hipStream_t s = (hipStream_t)stream;
+  const NormOp op{a, b, dtype_a, dtype_b};
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mu);
+    void* ws = nullptr;
+    rc = workspace((size_t)(np + 1) * sizeof(NormAcc), &ws);
+    if (rc) return rc;
+    rc = run_reduction(op, in, (NormAcc*)ws, (NormAcc*)ws + np, &host, s);
+  }
+  if (rc) return rc;
+  const double r[4] = {host.v[0], host.v[1], sqrt(host.v[2]), sqrt(host.v[3])};
+  for (int k = 0; k < nout; ++k) out[k] = host.v[4] != 0.0 ? __builtin_nan("") : r[k];
+  return ST_OK;
+}
+
 int st_diff_norms(const void* a, int dtype_a, const void* b, int dtype_b, int ndim,
                   const int64_t* shape, const int64_t* halo, void* stream, double* out) {
-  if (ndim < 1 || ndim > 4 || !out) return fail(ST_E_INVALID, "bad diff_norms arguments");
-  Shape4 sh;
-  for (int d = 0; d < 4; ++d) {
-    const int k = d - (4 - ndim);
-    sh.s[d] = k >= 0 ? shape[k] : 1;
-    sh.h[d] = k >= 0 ? halo[k] : 0;
-    sh.n[d] = sh.s[d] - 2 * sh.h[d];
-    if (sh.n[d] < 0) return fail(ST_E_INVALID, "negative interior");
-  }
-  hipStream_t s = (hipStream_t)stream;
-  static thread_local void* buf = nullptr;
-  static thread_local int buf_dev = -1;
-  int dev = 0;
-  HIPCHK(hipGetDevice(&dev));
-  if (!buf || buf_dev != dev) {
-    HIPCHK(hipMalloc(&buf, 3 * sizeof(unsigned long long)));
-    buf_dev = dev;
-  }
-  HIPCHK(hipMemsetAsync(buf, 0, 3 * sizeof(unsigned long long), s));
-  const int64_t total = sh.n[0] * sh.n[1] * sh.n[2] * sh.n[3];
-  unsigned g = grid_for(total);
-  if (g > 4096) g = 4096;
-  hipLaunchKernelGGL(k_diff_norms, dim3(g), dim3(256), 0, s, a, dtype_a, b, dtype_b, sh,
-                     (unsigned long long*)buf);
-  HIPCHK(hipGetLastError());
-  unsigned long long host[3];
-  HIPCHK(hipMemcpyAsync(host, buf, sizeof(host), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  memcpy(&out[0], &host[0], 8);
-  memcpy(&out[1], &host[1], 8);
-  if (host[2]) out[0] = out[1] = __builtin_nan("");
+  return norms_into(a, dtype_a, b, dtype_b, ndim, shape, halo, stream, out, 2);
+}
+
+int st_residual_norms(const void* a, int dtype_a, const void* b, int dtype_b, int ndim,
+                      const int64_t* shape, const int64_t* halo, void* stream, double* out) {
+  return norms_into(a, dtype_a, b, dtype_b, ndim, shape, halo, stream, out, 4);
+}
+
+int st_field_digest(const void* a, int dtype, int ndim, const int64_t* shape, const int64_t* halo, void* stream,
+                    uint64_t* out) {
+  if (!a || !out) return fail(ST_E_INVALID, "null argument");
+  Interior in;
+  int rc = make_interior(ndim, shape, halo, in);
+  if (rc) return rc;
+  const int64_t np = red_parts(in);
+  unsigned long long host = 0;
+  std::lock_guard<std::mutex> lock(g_ws_mu);
+  void* ws = nullptr;
+  rc = workspace((size_t)(np + 1) * sizeof(unsigned long long), &ws);
+  if (rc) return rc;
+  const DigestOp op{a, dtype};
+  rc = run_reduction(op, in, (unsigned long long*)ws, (unsigned long long*)ws + np, &host, (hipStream_t)stream);
+  if (rc) return rc;
+  *out = host;
   return ST_OK;
 }
 

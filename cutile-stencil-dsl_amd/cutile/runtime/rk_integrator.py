@@ -60,7 +60,19 @@ class RKRunner:
     def _pw(self, name, out, ins, dt):
         self.k[name]([out], ins, _STMT[name][2](dt))
 
-    def step(self, u, dt, launch_fn=None):
+    def step(self, u, dt, launch_fn=None, residual: bool = False):
+        """One step in place on ``u``.  With ``residual=True``, u is kept
+        before the step and ``self.last_residual`` holds the FP64 norms of
+        the step's change (``ResidualNorms``); the call then waits for the
+        current stream."""
+        if residual:
+            from cutile.runtime.norms import residual_norms
+
+            prev = self._buf(u, "prev")
+            prev.copy_(u)
+            self.step(u, dt, launch_fn)
+            self.last_residual = residual_norms(u, prev, self.r.halo_widths)
+            return u
         L = launch_fn or self.r.launcher
         m = self.method
         k1 = self._buf(u, "k1")

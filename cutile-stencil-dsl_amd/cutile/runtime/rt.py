@@ -24,8 +24,8 @@ BC_CODES = {None: 0, "none": 0, "dirichlet": 1, "neumann": 2, "periodic": 3, "re
 EXPORTS = (
     "st_last_error", "st_init", "st_device_count", "st_device_info_get", "st_mem_info",
     "st_compile_only", "st_free", "st_compile", "st_load_code", "st_unload", "st_get_function",
-    "st_launch", "st_function_resources", "st_fill_halo", "st_fill_halo_batched", "st_copy", "st_copy_wide", "st_diff_norms",
-    "st_bricks",
+    "st_launch", "st_function_resources", "st_fill_halo", "st_fill_halo_batched", "st_copy", "st_copy_wide",
+    "st_diff_norms", "st_residual_norms", "st_field_digest", "st_bricks",
     "st_event_create", "st_event_record", "st_event_elapsed_ms", "st_event_destroy", "st_event_sync",
     "st_stream_sync", "st_device_sync", "st_graph_begin", "st_graph_end", "st_graph_launch",
     "st_graph_destroy",
@@ -89,6 +89,10 @@ def _declare(lib):
         "st_copy_wide": ([vp, vp, ctypes.c_size_t, ctypes.c_int, vp], ctypes.c_int),
         "st_diff_norms": ([vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, i64p, i64p, vp,
                            ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "st_residual_norms": ([vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, i64p, i64p, vp,
+                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "st_field_digest": ([vp, ctypes.c_int, ctypes.c_int, i64p, i64p, vp, ctypes.POINTER(ctypes.c_uint64)],
+                            ctypes.c_int),
         "st_bricks": ([vp, vp, ctypes.c_int, ctypes.c_int, i64p, ctypes.c_int, ctypes.c_int, vp],
                       ctypes.c_int),
         "st_event_create": ([ctypes.POINTER(vp)], ctypes.c_int),
@@ -274,6 +278,23 @@ def diff_norms(a: int, da: int, b: int, db: int, shape, halo, stream=None) -> tu
     check(lib().st_diff_norms(a, da, b, db, len(shape), _i64(shape), _i64(halo), stream, out),
           "st_diff_norms")
     return out[0], out[1]
+
+
+def residual_norms(a: int, da: int, b: int, db: int, shape, halo, stream=None) -> tuple:
+    """(max|a-b|, max|b|, ||a-b||_2, ||b||_2) over the interior, in FP64
+    (``st_residual_norms``)."""
+    out = (ctypes.c_double * 4)()
+    check(lib().st_residual_norms(a, da, b, db, len(shape), _i64(shape), _i64(halo), stream, out),
+          "st_residual_norms")
+    return tuple(out)
+
+
+def field_digest(a: int, dtype_code: int, shape, halo, stream=None) -> int:
+    """Checksum of the interior's stored bits (``st_field_digest``)."""
+    out = ctypes.c_uint64(0)
+    check(lib().st_field_digest(a, dtype_code, len(shape), _i64(shape), _i64(halo), stream, ctypes.byref(out)),
+          "st_field_digest")
+    return int(out.value)
 
 
 def bricks(flat: int, bricked: int, dtype_code: int, shape, bs: int, to_bricks: bool, stream=None) -> None:

@@ -180,10 +180,15 @@ class TimeStepper:
             self.fill(a, stream)          # wrapped launches left the halo stale
         return a, b
 
-    def march(self, u, steps: int, scratch=None, stream=None, graph: bool = False, graph_steps: int = 0):
+    def march(self, u, steps: int, scratch=None, stream=None, graph: bool = False, graph_steps: int = 0,
+              residual: bool = False):
         """Run ``steps`` steps starting from device array ``u`` (its halo is
         filled first).  With ``graph=True`` the loop is captured once (or
-        ``graph_steps`` steps of it, replayed) as a hipGraph."""
+        ``graph_steps`` steps of it, replayed) as a hipGraph.  With
+        ``residual=True`` the last step runs on its own (not inside a
+        temporal block, the same state) and ``self.last_residual`` holds the
+        FP64 norms of its change (``cutile.runtime.norms.ResidualNorms``);
+        the call then waits for the stream."""
         import torch
 
         from cutile.runtime.kernel import default_stream
@@ -192,6 +197,18 @@ class TimeStepper:
             stream = default_stream()
         b = scratch if scratch is not None else torch.empty_like(u)
         self.fill(u, stream)
+        if residual and steps > 0:
+            from cutile.runtime.norms import residual_norms
+
+            head = u
+            if steps > 1:
+                head = self.march(u, steps - 1, b, stream, graph, graph_steps)
+                other = b if head is u else u
+            else:
+                other = b
+            new, old = self.enqueue(head, other, 1, stream)
+            self.last_residual = residual_norms(new, old, self.halo, stream)
+            return new
         if not graph:
             return self.enqueue(u, b, steps, stream)[0]
         # graphs are captured on a private stream ordered after the caller's
@@ -265,11 +282,28 @@ class FusedStepper:
             if kind in ("periodic", "dirichlet0"):
                 self._blk, self.T = block.load_module()._KERNEL, int(block.temporal_steps)
 
-    def enqueue(self, fields: dict, scratch: dict, steps: int, stream=None):
+    def enqueue(self, fields: dict, scratch: dict, steps: int, stream=None, residual: bool = False):
+        """Advance ``steps`` fused steps (fields <-> scratch swap per step).
+        With ``residual=True`` the last step runs as a single fused step and
+        ``self.last_residual`` maps each target to the FP64 norms of its
+        change (``ResidualNorms``); the call then waits for the stream."""
         from cutile.runtime.kernel import default_stream
 
         if stream is None:
             stream = default_stream()
+        if residual and steps > 0:
+            from cutile.runtime.norms import residual_norms
+
+            if steps > 1:
+                fields, scratch = self.enqueue(fields, scratch, steps - 1, stream)
+            blk, self._blk = self._blk, None
+            try:
+                fields, scratch = self.enqueue(fields, scratch, 1, stream)
+            finally:
+                self._blk = blk
+            self.last_residual = {t: residual_norms(fields[t], scratch[t], self.halo, stream)
+                                  for t in self.targets}
+            return fields, scratch
         if self.policy.mode == "static":
             for t in self.fill_names:
                 _boundary.fill(scratch[t], self.spec, self.halo, stream)

@@ -21,6 +21,8 @@
  *   to_bricks / from_bricks (multigpu_emitter.py:370-434) -> st_bricks
  *   max-norm comparisons in validate()               -> st_diff_norms
  *     (runtime/launcher.py:127-131)
+ *   (no upstream counterpart: FP64 residual norms, field checksums)
+ *                                                    -> st_residual_norms, st_field_digest
  *   (no upstream counterpart: hipGraph capture of step loops)
  *                                                    -> st_graph_*
  *
@@ -149,9 +151,29 @@ int st_copy_wide(void* dst, const void* src, size_t nbytes, int vectors, void* s
 
 /* Max-norms over the interior of two arrays of the same shape:
  * out[0] = max|a-b|, out[1] = max|b| (computed in FP64). `out` is host memory;
- * the call synchronizes the stream. */
+ * the call synchronizes the stream.  Replaces the max-norm comparison of
+ * validate() (runtime/launcher.py:127-131).  The reduction runs in two
+ * passes (partials of at most 65536 cells, then one workgroup) in a
+ * workspace the library allocates once per device and grows on demand. */
 int st_diff_norms(const void* a, int dtype_a, const void* b, int dtype_b, int ndim,
                   const int64_t* shape, const int64_t* halo, void* stream, double* out);
+
+/* Residual norms of a time step (no upstream counterpart; the stepping API's
+ * FP64 residuals): over the interior, out[0] = max|a-b|, out[1] = max|b|,
+ * out[2] = ||a-b||_2, out[3] = ||b||_2, all accumulated in FP64 with a
+ * summation order fixed by the shape (reproducible bit for bit).  NaN in
+ * either array gives NaN in every output.  Synchronizes the stream. */
+int st_residual_norms(const void* a, int dtype_a, const void* b, int dtype_b, int ndim,
+                      const int64_t* shape, const int64_t* halo, void* stream, double* out);
+
+/* Checksum of an array's interior (no upstream counterpart; pins device
+ * states against the oracle at sizes where copying them out is too slow):
+ * the sum modulo 2^64 over interior cells e (row-major interior index) of
+ * mix64(bits(x_e) XOR e * 0x9e3779b97f4a7c15), where bits are the stored
+ * bits of the element (dtype ST_F64 / ST_F32 / ST_BF16, zero-extended) and
+ * mix64 is the SplitMix64 finaliser.  Synchronizes the stream. */
+int st_field_digest(const void* a, int dtype, int ndim, const int64_t* shape, const int64_t* halo, void* stream,
+                    uint64_t* out);
 
 /* Flat <-> bricked layout (to_bricks / from_bricks): `flat` has `shape`,
  * `bricked` is the brick-major array of shape (nb0, .., nb_{d-1}, bs, .., bs)
