@@ -414,6 +414,31 @@ int workspace(size_t bytes, void** out) {
   return ST_OK;
 }
 
+// The allocation-free variant's scratch: a fixed array in this library's
+// device image, one copy per GPU.  Host code reaches it only through the
+// per-device address from hipGetSymbolAddress (taken in st_init, or on the
+// first call on a device); the array's name is not a device address on the
+// host.
+constexpr int64_t STATIC_PARTS = 16384;
+static __device__ __align__(256) NormAcc scratch[STATIC_PARTS + 1];
+std::mutex g_scratch_mu;
+std::map<int, NormAcc*> g_scratch;  // device -> address of `scratch` on it
+
+int scratch_address(int dev, NormAcc** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch.find(dev);
+  if (it == g_scratch.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail((int)(e != hipSuccess ? e : hipErrorInvalidSymbol),
+                  "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    it = g_scratch.emplace(dev, (NormAcc*)p).first;
+  }
+  *out = it->second;
+  return ST_OK;
+}
+
 template <class E>
 __global__ void k_bricks(E* flat, E* bricked, int ndim, Shape4 sh, int64_t nb0, int64_t nb1,
                          int64_t nb2, int bs, int to_bricks, int64_t total) {
@@ -460,7 +485,8 @@ const char* st_last_error(void) { return g_err.c_str(); }
 
 int st_init(int dev) {
   HIPCHK(hipSetDevice(dev));
-  return ST_OK;
+  NormAcc* p = nullptr;
+  return scratch_address(dev, &p);
 }
 
 int st_device_count(int* count) {
@@ -730,7 +756,7 @@ int st_copy_wide(void* dst, const void* src, size_t nbytes, int vectors, void* s
 }
 
 static int norms_into(const void* a, int dtype_a, const void* b, int dtype_b, int ndim, const int64_t* shape,
-                      const int64_t* halo, void* stream, double* out, int nout) {
+                      const int64_t* halo, void* stream, double* out, int nout, bool no_alloc) {
   if (!a || !b || !out) return fail(ST_E_INVALID, "null argument");
   Interior in;
   int rc = make_interior(ndim, shape, halo, in);
@@ -739,7 +765,21 @@ static int norms_into(const void* a, int dtype_a, const void* b, int dtype_b, in
   NormAcc host{};
   hipStream_t s = (hipStream_t)stream;
   const NormOp op{a, b, dtype_a, dtype_b};
-  {
+  if (no_alloc) {
+    if (np > STATIC_PARTS)
+      return fail(ST_E_INVALID,
+                  "st_residual_norms_static: the interior needs %lld partials of at most %lld cells, more than "
+                  "the %lld the static scratch holds (fields up to about %lld interior cells); use "
+                  "st_residual_norms",
+                  (long long)np, (long long)RED_CELLS, (long long)STATIC_PARTS,
+                  (long long)(STATIC_PARTS * RED_CELLS));
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    NormAcc* sp = nullptr;
+    rc = scratch_address(dev, &sp);
+    if (rc) return rc;
+    rc = run_reduction(op, in, sp, sp + STATIC_PARTS, &host, s);
+  } else {
     std::lock_guard<std::mutex> lock(g_ws_mu);
     void* ws = nullptr;
     rc = workspace((size_t)(np + 1) * sizeof(NormAcc), &ws);
@@ -754,12 +794,17 @@ static int norms_into(const void* a, int dtype_a, const void* b, int dtype_b, in
 
 int st_diff_norms(const void* a, int dtype_a, const void* b, int dtype_b, int ndim,
                   const int64_t* shape, const int64_t* halo, void* stream, double* out) {
-  return norms_into(a, dtype_a, b, dtype_b, ndim, shape, halo, stream, out, 2);
+  return norms_into(a, dtype_a, b, dtype_b, ndim, shape, halo, stream, out, 2, false);
 }
 
 int st_residual_norms(const void* a, int dtype_a, const void* b, int dtype_b, int ndim,
                       const int64_t* shape, const int64_t* halo, void* stream, double* out) {
-  return norms_into(a, dtype_a, b, dtype_b, ndim, shape, halo, stream, out, 4);
+  return norms_into(a, dtype_a, b, dtype_b, ndim, shape, halo, stream, out, 4, false);
+}
+
+int st_residual_norms_static(const void* a, int dtype_a, const void* b, int dtype_b, int ndim,
+                             const int64_t* shape, const int64_t* halo, void* stream, double* out) {
+  return norms_into(a, dtype_a, b, dtype_b, ndim, shape, halo, stream, out, 4, true);
 }
 
 int st_field_digest(const void* a, int dtype, int ndim, const int64_t* shape, const int64_t* halo, void* stream,

@@ -30,10 +30,12 @@ class ResidualNorms:
         return self.l2 / self.l2_ref if self.l2_ref > 0 else self.l2
 
 
-def residual_norms(new, old, halo, stream=None) -> ResidualNorms:
+def residual_norms(new, old, halo, stream=None, no_alloc: bool = False) -> ResidualNorms:
     """Norms of ``new - old`` relative to ``new`` over the interior of two
     device arrays of the same shape (halo ``halo``); synchronizes
-    ``stream`` (default: torch's current stream)."""
+    ``stream`` (default: torch's current stream).  ``no_alloc`` uses the
+    runtime's fixed scratch (``st_residual_norms_static``) instead of its
+    workspace: no device allocation, interiors up to about 1.07e9 cells."""
     from cutile.runtime import rt
     from cutile.runtime.kernel import default_stream, dtype_code
 
@@ -46,5 +48,5 @@ def residual_norms(new, old, halo, stream=None) -> ResidualNorms:
             raise ValueError("residual_norms: arrays must be C-contiguous device tensors")
     s = default_stream() if stream is None else stream
     out = rt.residual_norms(old.data_ptr(), dtype_code(old), new.data_ptr(), dtype_code(new), tuple(new.shape),
-                            tuple(halo), s)
+                            tuple(halo), s, no_alloc=no_alloc)
     return ResidualNorms(*out)

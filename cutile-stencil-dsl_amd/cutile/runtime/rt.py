@@ -25,7 +25,7 @@ EXPORTS = (
     "st_last_error", "st_init", "st_device_count", "st_device_info_get", "st_mem_info",
     "st_compile_only", "st_free", "st_compile", "st_load_code", "st_unload", "st_get_function",
     "st_launch", "st_function_resources", "st_fill_halo", "st_fill_halo_batched", "st_copy", "st_copy_wide",
-    "st_diff_norms", "st_residual_norms", "st_field_digest", "st_bricks",
+    "st_diff_norms", "st_residual_norms", "st_residual_norms_static", "st_field_digest", "st_bricks",
     "st_event_create", "st_event_record", "st_event_elapsed_ms", "st_event_destroy", "st_event_sync",
     "st_stream_sync", "st_device_sync", "st_graph_begin", "st_graph_end", "st_graph_launch",
     "st_graph_destroy",
@@ -91,6 +91,8 @@ def _declare(lib):
                            ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "st_residual_norms": ([vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, i64p, i64p, vp,
                                ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "st_residual_norms_static": ([vp, ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, i64p, i64p, vp,
+                                      ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "st_field_digest": ([vp, ctypes.c_int, ctypes.c_int, i64p, i64p, vp, ctypes.POINTER(ctypes.c_uint64)],
                             ctypes.c_int),
         "st_bricks": ([vp, vp, ctypes.c_int, ctypes.c_int, i64p, ctypes.c_int, ctypes.c_int, vp],
@@ -280,12 +282,13 @@ def diff_norms(a: int, da: int, b: int, db: int, shape, halo, stream=None) -> tu
     return out[0], out[1]
 
 
-def residual_norms(a: int, da: int, b: int, db: int, shape, halo, stream=None) -> tuple:
+def residual_norms(a: int, da: int, b: int, db: int, shape, halo, stream=None, no_alloc: bool = False) -> tuple:
     """(max|a-b|, max|b|, ||a-b||_2, ||b||_2) over the interior, in FP64
-    (``st_residual_norms``)."""
+    (``st_residual_norms``; ``no_alloc``: ``st_residual_norms_static``)."""
     out = (ctypes.c_double * 4)()
-    check(lib().st_residual_norms(a, da, b, db, len(shape), _i64(shape), _i64(halo), stream, out),
-          "st_residual_norms")
+    fn = lib().st_residual_norms_static if no_alloc else lib().st_residual_norms
+    check(fn(a, da, b, db, len(shape), _i64(shape), _i64(halo), stream, out),
+          "st_residual_norms_static" if no_alloc else "st_residual_norms")
     return tuple(out)
 
 

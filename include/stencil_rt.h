@@ -22,7 +22,7 @@
  *   max-norm comparisons in validate()               -> st_diff_norms
  *     (runtime/launcher.py:127-131)
  *   (no upstream counterpart: FP64 residual norms, field checksums)
- *                                                    -> st_residual_norms, st_field_digest
+ *                                                    -> st_residual_norms(_static), st_field_digest
  *   (no upstream counterpart: hipGraph capture of step loops)
  *                                                    -> st_graph_*
  *
@@ -165,6 +165,18 @@ int st_diff_norms(const void* a, int dtype_a, const void* b, int dtype_b, int nd
  * either array gives NaN in every output.  Synchronizes the stream. */
 int st_residual_norms(const void* a, int dtype_a, const void* b, int dtype_b, int ndim,
                       const int64_t* shape, const int64_t* halo, void* stream, double* out);
+
+/* st_residual_norms without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  The partials live in a fixed
+ * array in this library's device image (16385 x 40 B = 640 KiB per GPU,
+ * held until the process ends): interiors needing more than 16384 partials
+ * of 65536 cells (about 1.07e9 cells) are rejected with ST_E_INVALID.
+ * Every call on a GPU shares that GPU's array, so calls that overlap on one
+ * GPU (other threads, other streams) race; serialize them.  Same outputs,
+ * bit for bit, as st_residual_norms. */
+int st_residual_norms_static(const void* a, int dtype_a, const void* b, int dtype_b, int ndim,
+                             const int64_t* shape, const int64_t* halo, void* stream, double* out);
 
 /* Checksum of an array's interior (no upstream counterpart; pins device
  * states against the oracle at sizes where copying them out is too slow):
