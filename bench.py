@@ -204,7 +204,7 @@ def temporal_depths(interior_dev, n, steps=50):
             first = None
             for T in (1, 2, 4):
                 key = f"{sname}_{dt_name}_T{T}"
-                st = TimeStepper(mk(dt_name), periodic_spec(3), temporal_steps=T)
+                st = TimeStepper(mk(dt_name), periodic_spec(3), temporal_steps=T, split="fit")
                 u = st.empty_like_interior((n, n, n))
                 inner = tuple(slice(h, h + n) for h in st.halo)
                 u[inner] = interior_dev.to(u.dtype)
@@ -229,6 +229,15 @@ def temporal_depths(interior_dev, n, steps=50):
                     f"({kind}; state after {steps} steps equal to T=1: {same})")
                 del u, st, g, state
                 torch.cuda.empty_cache()
+            # the depth split the autotuner picks for a block of 4 steps
+            st = TimeStepper(mk(dt_name), periodic_spec(3), temporal_steps=4, split="auto")
+            u = st.empty_like_interior((n, n, n))
+            kern, reps = st.plan_for(u)
+            depth = 1 if kern is None else 4 // reps
+            out[f"{sname}_{dt_name}_T4_tuned_split"] = f"{4 // depth} x depth {depth}"
+            log(f"{sname} {dt_name}: autotuned split of a 4-step block: {4 // depth} x depth {depth}")
+            del u, st
+            torch.cuda.empty_cache()
             del first
     return out
 
@@ -250,8 +259,8 @@ def heat2d_c1(steps=100, n=1024):
 
     spec = dirichlet_spec(2)
     ref, out, cpu_s = None, {}, None
-    for T in (1, 4, 10):
-        st = TimeStepper(heat2d(), spec, temporal_steps=T)
+    for T, split in ((1, "fit"), (4, "fit"), (10, "fit"), (4, "auto"), (10, "auto")):
+        st = TimeStepper(heat2d(), spec, temporal_steps=T, split=split)
         u0 = heat2d_init(n, st.halo, seed=0)
         if ref is None or ref[0] != st.halo:
             t0 = time.perf_counter()
@@ -267,10 +276,14 @@ def heat2d_c1(steps=100, n=1024):
         torch.cuda.synchronize()
         I = oracle.interior(u0.shape, st.halo)
         exact = bool(np.array_equal(got[I], ref[1][I]))
-        out[f"T{T}"] = {"ms_per_step": ms / steps, "gcell_updates_per_s": n * n * steps / (ms * 1e-3) / 1e9,
-                        "bit_exact_vs_numpy": exact, "halo": list(st.halo)}
-        log(f"heat2d 1024^2 x{steps} T={T}: {out[f'T{T}']['gcell_updates_per_s']:.1f} Gcell-updates/s, "
-            f"bit-exact={exact}")
+        name = f"T{T}" if split == "fit" else f"T{T}_tuned"
+        kern, reps = st.plan_for(u) if T > 1 else (None, 1)
+        depth = 1 if kern is None else T // reps
+        out[name] = {"ms_per_step": ms / steps, "gcell_updates_per_s": n * n * steps / (ms * 1e-3) / 1e9,
+                     "bit_exact_vs_numpy": exact, "halo": list(st.halo),
+                     "launches": f"{T // depth} x depth {depth} per block of {T}"}
+        log(f"heat2d 1024^2 x{steps} T={T} ({split}: {out[name]['launches']}): "
+            f"{out[name]['gcell_updates_per_s']:.1f} Gcell-updates/s, bit-exact={exact}")
         del u, res, st, g
     out["cpu_numpy_gcell_updates_per_s"] = n * n * steps / cpu_s / 1e9
     log(f"heat2d NumPy: {out['cpu_numpy_gcell_updates_per_s']:.3f} Gcell-updates/s")
@@ -400,11 +413,13 @@ def gray_scott_c3(n=4096, steps=1000):
     return out
 
 
-def batched_wave_c4(count=512, steps=200, seed=0):
+def batched_wave_c4(count=512, steps=200, seed=0, graph=True):
     """512 leapfrog wave instances (sides log-uniform in [64, 2048], Gaussian
     pulse, zero Dirichlet): the batched float32 persistent kernel (graph of a
     full buffer rotation) against plain float64 one-launch-per-instance
-    (also graph-captured), same step count; error of float32 vs float64."""
+    (also graph-captured), same step count; error of float32 vs float64.
+    ``graph=False`` launches every step eagerly (for profilers whose
+    counter collection does not follow graph launches)."""
     import math
 
     import numpy as np
@@ -447,11 +462,12 @@ def batched_wave_c4(count=512, steps=200, seed=0):
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
-        g = rt.Graph(side.cuda_stream)
-        with g:
-            for _ in range(3):
-                lf.step(side.cuda_stream)
-        reps, rest = divmod(steps, 3)
+        g = rt.Graph(side.cuda_stream) if graph else None
+        if graph:
+            with g:
+                for _ in range(3):
+                    lf.step(side.cuda_stream)
+        reps, rest = divmod(steps, 3) if graph else (0, steps)
         def run32():
             for _ in range(reps):
                 g.launch()
@@ -481,10 +497,11 @@ def batched_wave_c4(count=512, steps=200, seed=0):
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
-        g64 = rt.Graph(side.cuda_stream)
-        with g64:
-            for k in range(3):
-                step64(k, side.cuda_stream)
+        g64 = rt.Graph(side.cuda_stream) if graph else None
+        if graph:
+            with g64:
+                for k in range(3):
+                    step64(k, side.cuda_stream)
         def run64():
             for _ in range(reps):
                 g64.launch()
@@ -520,7 +537,7 @@ def serving_c5(seconds=60.0, rate=200.0, seed=0):
     from cutile.serving import StencilServer, generate_jobs, warmup_jobs
 
     srv = StencilServer()
-    srv.run(warmup_jobs(), realtime=False)
+    srv.run(warmup_jobs(srv), realtime=False)
     jobs = generate_jobs(rate, seconds, seed)
     stats = srv.run(jobs, realtime=True, log=log)
     stats.update(rate=rate, seconds=seconds, offered_jobs=len(jobs),
@@ -575,6 +592,8 @@ def main():
                     help="extras limited to the other 3-D sweeps and the copy reference (for profiling)")
     ap.add_argument("--only", choices=["temporal", "heat2d", "gray_scott", "batched", "serving"],
                     help="the headline plus this one extra section (for profiling)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched section: eager launches instead of graph replays (for counter collection)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py drives one GPU: this build runs every workload on a single MI355X "
@@ -604,7 +623,8 @@ def main():
         args.no_extra = True
         sect = {"temporal": ("temporal_depths_c2", lambda: temporal_depths(interior, n)),
                 "heat2d": ("heat2d_c1", heat2d_c1), "gray_scott": ("gray_scott_c3", gray_scott_c3),
-                "batched": ("batched_wave_c4", batched_wave_c4), "serving": ("serving_c5", serving_c5)}
+                "batched": ("batched_wave_c4", lambda: batched_wave_c4(graph=not args.no_graph)),
+                "serving": ("serving_c5", serving_c5)}
         key, fn = sect[args.only]
         extra[key] = fn()
     if not args.no_extra:

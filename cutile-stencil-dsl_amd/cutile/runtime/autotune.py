@@ -277,12 +277,18 @@ def autotune(stencil_fn, domain=None, hw=None, max_candidates: int = 30, warmup:
 # batched kernel: tiles per size class
 # ---------------------------------------------------------------------------
 
-def bucket_candidates(vz: int, max_cols: Optional[int]) -> list:
-    """Bucket shapes tried for 2-D instances whose rows hold at most
-    ``max_cols`` cells (``None``: the open-ended largest class)."""
+def bucket_candidates(vz: int, max_cols: Optional[int], ndim: int = 2) -> list:
+    """Bucket shapes tried for instances whose rows hold at most
+    ``max_cols`` cells (``None``: the open-ended largest class): 2-D
+    plane-mapped tiles and row tiles, or 3-D row tiles streaming along the
+    first axis."""
     from cutile.runtime.batched import Bucket
 
     out = []
+    if ndim == 3:
+        for ty, nz, xch in ((8, 32, 16), (8, 64, 32), (4, 64, 16), (16, 32, 32), (4, 32, 8)):
+            out.append(Bucket("rows", ty, nz * vz, xch, max_cols))
+        return out
     for ty, tz in ((16, 16 * vz), (8, 32 * vz), (4, 64 * vz), (16, 32 * vz)):
         out.append(Bucket("plane", ty, tz, 1, max_cols))
     for tz, xch in ((256 * vz, 16), (256 * vz, 64), (128 * vz, 32)):
@@ -304,13 +310,16 @@ def autotune_buckets(stencil_fn, sides: Sequence[int], storage: Optional[str] = 
 
     probe = BatchedStencil(stencil_fn, storage=storage)
     vz = probe.vz
+    nd = probe.ndim
     defaults = probe.buckets
     chosen = []
     lo = 0
     for k, bound in enumerate(bounds):
         members = [s for s in sides if s > lo and (bound is None or s <= bound)]
         lo = bound if bound is not None else lo
-        cands = bucket_candidates(vz, bound)
+        if nd == 3 and len(members) > 8:              # 3-D instances are large: a spread of 8
+            members = sorted(members)[:: max(1, len(members) // 8)][:8]
+        cands = bucket_candidates(vz, bound, nd)
         if not members or not torch.cuda.is_available():
             d = defaults[min(k, len(defaults) - 1)]
             chosen.append(Bucket(d.mapping, d.ty, d.tz, d.xch, bound))
@@ -319,15 +328,15 @@ def autotune_buckets(stencil_fn, sides: Sequence[int], storage: Optional[str] = 
         arrays = []
         for s in members[:64]:
             cols = -(-s // vz) * vz
-            shp = (s + 2 * halo, cols + 2 * max(halo, vz))
+            shp = (s + 2 * halo,) * (nd - 1) + (cols + 2 * max(halo, vz),)
             ins = [torch.rand(shp, device="cuda", dtype=torch.float32).to(dt) for _ in range(probe.num_inputs)]
             outs = [torch.zeros(shp, device="cuda", dtype=dt) for _ in range(probe.num_outputs)]
-            arrays.append((ins, outs, (s, s)))
+            arrays.append((ins, outs, (s,) * nd))
         best = None
         for b in cands:
             try:
                 bs = BatchedStencil(stencil_fn, storage=storage, buckets=[Bucket(b.mapping, b.ty, b.tz, b.xch, None)])
-                plan = bs.prepare([(i, o) for i, o, _ in arrays], (halo, max(halo, vz)),
+                plan = bs.prepare([(i, o) for i, o, _ in arrays], (halo,) * (nd - 1) + (max(halo, vz),),
                                   interiors=[n for _, _, n in arrays])
                 s = default_stream()
                 bs.launch(plan, s)
@@ -350,3 +359,185 @@ def autotune_buckets(stencil_fn, sides: Sequence[int], storage: Optional[str] = 
         else:
             chosen.append(best[1])
     return chosen
+
+
+# ---------------------------------------------------------------------------
+# temporal depth: how a block of T steps is split into launches
+# ---------------------------------------------------------------------------
+
+def split_candidates(T: int) -> list:
+    """Depths d dividing T (d = T: one launch per block; d = 1: T sweeps)."""
+    return [d for d in range(T, 0, -1) if T % d == 0]
+
+
+def autotune_split(stepper, u, min_ms: float = 20.0, verbose: bool = False) -> int:
+    """The depth d (a divisor of ``stepper.T``; 1 = single sweeps) whose
+    launches advance arrays shaped like ``u`` fastest per step.  Each
+    candidate runs whole blocks of T steps on two scratch arrays of u's
+    shape (zeros, the stepper's boundary handling) for at least ``min_ms``;
+    depths whose levels fit neither registers nor LDS are skipped.  The
+    choice is cached per (stencil, storage, size bucket, T, mode, arch) in
+    the autotune cache.  Every candidate gives the same states."""
+    import torch
+
+    from cutile.runtime import rt
+    from cutile.runtime.kernel import current_arch, default_stream
+
+    T = stepper.T
+    interior = tuple(int(n) - 2 * h for n, h in zip(u.shape, stepper.halo))
+    arch = current_arch().split(":")[0]
+    key = cache_key(stepper.fn, stepper.storage, interior, ("split", T, stepper.mode), arch)
+    hit = _load(key)
+    if hit is not None:
+        return int(hit.temporal_steps)
+    a = torch.zeros_like(u)
+    b = torch.zeros_like(u)
+    s = default_stream()
+    best = None
+    saved = dict(stepper._plans)
+    for d in split_candidates(T):
+        kern = None if d == 1 else stepper.depth_kernel(d)
+        if d > 1 and kern is None:
+            continue
+        stepper._plans[(tuple(u.shape), u.dtype)] = (kern, T // d)
+        try:
+            x, y = stepper.enqueue(a, b, T, s)              # build and warm up
+            torch.cuda.synchronize()
+            reps, ms = 0, 0.0
+            while ms < min_ms and reps < 64:
+                e0, e1 = rt.Event(), rt.Event()
+                e0.record(s)
+                x, y = stepper.enqueue(x, y, T, s)
+                e1.record(s)
+                ms += e0.elapsed_ms(e1)
+                reps += 1
+        except (ValueError, RuntimeError):
+            continue
+        per_step = ms / (reps * T)
+        if verbose:
+            print(f"  split T={T} as {T // d} x depth {d}: {per_step * 1e3:.2f} us/step")
+        if best is None or per_step < best[0]:
+            best = (per_step, d)
+    stepper._plans.clear()
+    stepper._plans.update(saved)
+    del a, b
+    d = best[1] if best is not None else T // stepper.block_reps
+    cells = math.prod(interior)
+    ms = best[0] if best is not None else 0.0
+    _save(key, AutotuneResult(tuple(interior), d, cells / (ms * 1e-3) / 1e9 if ms else 0.0, 0.0, ms, {}))
+    return d
+
+
+# ---------------------------------------------------------------------------
+# tuned tables: measured choices kept per (what, stencil, storage, sizes, arch)
+# ---------------------------------------------------------------------------
+#
+# A measured choice is looked up first in the user's autotune cache, then in
+# the table shipped with the package (``cutile/tuned/<arch>.json``, written
+# by ``export_tuned`` from a tuning run on that architecture); a miss with a
+# device present is measured and cached.  Without a device and without an
+# entry the caller's default applies.
+
+def tuned_path(arch: str) -> str:
+    return os.path.join(os.path.dirname(os.path.dirname(__file__)), "tuned", f"{arch}.json")
+
+
+def _parts_key(parts: dict) -> str:
+    return hashlib.sha1(json.dumps(parts, sort_keys=True).encode()).hexdigest()
+
+
+def _shipped(arch: str) -> dict:
+    path = tuned_path(arch)
+    if not os.path.exists(path):
+        return {}
+    try:
+        return {_parts_key(e["key"]): e["value"] for e in json.load(open(path))["entries"]}
+    except (OSError, ValueError, KeyError):
+        return {}
+
+
+def tuned(parts: dict, measure, arch: str):
+    """The cached value of the measured choice described by ``parts`` (a
+    JSON-able dict), measuring it with ``measure()`` on a miss when a device
+    is present; None when there is neither an entry nor a device."""
+    import torch
+
+    parts = dict(parts, arch=arch, version=_CACHE_VERSION)
+    key = _parts_key(parts)
+    root = cache_root()
+    path = os.path.join(root, f"tuned-{key}.json") if root else ""
+    if path and os.path.exists(path):
+        try:
+            return json.load(open(path))["value"]
+        except (OSError, ValueError, KeyError):
+            pass
+    hit = _shipped(arch).get(key)
+    if hit is not None:
+        return hit
+    if not torch.cuda.is_available():
+        return None
+    value = measure()
+    if path and value is not None:
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump({"key": parts, "value": value}, f)
+        os.replace(tmp, path)
+    return value
+
+
+def export_tuned(arch: str, path: Optional[str] = None) -> str:
+    """Merge every tuned entry of the user cache into the shipped table for
+    ``arch`` (the package's ``tuned/<arch>.json`` unless ``path``)."""
+    import glob
+
+    path = path or tuned_path(arch)
+    entries = {}
+    if os.path.exists(path):
+        for e in json.load(open(path))["entries"]:
+            entries[_parts_key(e["key"])] = e
+    for f in glob.glob(os.path.join(cache_root(), "tuned-*.json")):
+        e = json.load(open(f))
+        if e["key"].get("arch") == arch:
+            entries[_parts_key(e["key"])] = e
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump({"arch": arch, "entries": sorted(entries.values(), key=lambda e: json.dumps(e["key"],
+                                                                                               sort_keys=True))},
+                  f, indent=1)
+    return path
+
+
+def _arch() -> str:
+    import torch
+
+    if torch.cuda.is_available():
+        from cutile.runtime.kernel import current_arch
+
+        return current_arch().split(":")[0]
+    return "gfx950"
+
+
+def tuned_buckets(stencil_fn, sides: Sequence[int], storage: Optional[str] = None,
+                  bounds: Sequence[Optional[int]] = (160, 384, None), halo: int = 1) -> Optional[list]:
+    """Tile buckets of the batched kernel for instances with these ``sides``
+    (one per size class of ``bounds``), from the tuned tables or measured
+    with ``autotune_buckets``; None when neither is available (the
+    kernel's defaults then apply).  Keyed by the stencil, its storage, the
+    class bounds, the power-of-two size buckets of the sides and the arch."""
+    from cutile.frontend.ir import find_func
+    from cutile.runtime.batched import Bucket
+
+    fns = list(stencil_fn) if isinstance(stencil_fn, (list, tuple)) else [stencil_fn]
+    func = find_func(fns[0]._ir)
+    parts = {"what": "batched_buckets", "stencil": [f._fn.__name__ for f in fns],
+             "ir": [_ir_digest(f) for f in fns], "storage": storage or func.dtype,
+             "bounds": [b for b in bounds], "sizes": sorted({size_bucket([int(s)])[0] for s in sides})}
+
+    def measure():
+        got = autotune_buckets(stencil_fn, sides, storage=storage, bounds=bounds, halo=halo)
+        return [[b.mapping, b.ty, b.tz, b.xch, b.max_cols] for b in got]
+
+    val = tuned(parts, measure, _arch())
+    if val is None:
+        return None
+    return [Bucket(m, int(ty), int(tz), int(xch), mc) for m, ty, tz, xch, mc in val]

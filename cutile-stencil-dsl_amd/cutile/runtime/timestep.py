@@ -71,11 +71,27 @@ class TimeStepper:
     by ``apply_boundary``."""
 
     def __init__(self, stencil_fn, boundary, temporal_steps: int = 1, storage: Optional[str] = None,
-                 tile_sizes=None, kernel_hints: Optional[dict] = None, block_hints: Optional[dict] = None):
+                 tile_sizes=None, kernel_hints: Optional[dict] = None, block_hints: Optional[dict] = None,
+                 split: str = "auto"):
+        """``split``: how a block of ``temporal_steps`` steps is launched.
+        ``"fit"``: one launch of depth T, or, where T levels do not fit the
+        registers and LDS, T/d launches of the largest depth d dividing T
+        that fits.  ``"auto"`` (default): on the first array of a given
+        shape, every depth d dividing T (d = 1: single sweeps) is timed on
+        scratch arrays of that shape and the fastest is used for that shape
+        (the states are the same for every d; cached per stencil, storage,
+        size bucket, T and device architecture, see
+        ``cutile.runtime.autotune.autotune_split``).  Without a device
+        ``"auto"`` behaves as ``"fit"``."""
         from cutile.runtime.launcher import compile as _compile
 
+        if split not in ("auto", "fit"):
+            raise ValueError(f"split must be 'auto' or 'fit', got {split!r}")
         self.fn = stencil_fn
         self.spec = boundary
+        self.split = split
+        self._storage_arg, self._block_hints = storage, block_hints
+        self._plans: dict = {}
         self.T = int(temporal_steps)
         from cutile.frontend.ir import find_func
 
@@ -133,11 +149,51 @@ class TimeStepper:
             self.fill(b, stream)
         return b, a
 
+    # -- depth split by measured speed ------------------------------------
+    def depth_kernel(self, d: int):
+        """Compiled block kernel of depth ``d`` (a divisor of T, >= 2) in this
+        stepper's mode, or None when d levels fit neither registers nor LDS."""
+        from cutile.runtime.launcher import compile as _compile
+
+        if d == self.T // self.block_reps and self._block is not None:
+            return self._block
+        r = _compile(self.fn, temporal_steps=d, temporal_mode=self.mode, storage=self._storage_arg,
+                     kernel_hints=self._block_hints)
+        k = r.load_module()._KERNEL
+        return k if "tblock" in k.variants else None
+
+    def plan_for(self, u) -> tuple:
+        """(kernel, launches per block) used for arrays shaped like ``u``:
+        the fit-based choice, or under ``split="auto"`` the measured fastest
+        (kernel None: T single sweeps)."""
+        key = (tuple(u.shape), u.dtype)
+        plan = self._plans.get(key)
+        if plan is None:
+            plan = (self._block, self.block_reps)
+            if self.split == "auto" and self._block is not None:
+                import torch
+
+                if torch.cuda.is_available() and u.is_cuda:
+                    from cutile.runtime.autotune import autotune_split
+
+                    d = autotune_split(self, u)
+                    plan = (None, self.T) if d == 1 else (self.depth_kernel(d), self.T // d)
+            self._plans[key] = plan
+        return plan
+
     def _blk(self, a, b, stream, wrap=()):
         # the halo of `a` is current (every step or block ends with a fill),
         # or `wrap` names the periodic axes the block reads around
-        for _ in range(self.block_reps):
-            self._block.launch_temporal([a], [b], self.halo, stream,
+        kern, reps = self.plan_for(a)
+        if kern is None:                      # measured fastest: T single sweeps
+            axes = self.policy.axes_for(a)
+            if wrap and not axes:
+                self.fill(a, stream)          # earlier wrapped launches left it stale
+            for _ in range(self.T):
+                a, b = self._one(a, b, stream, axes)
+            return a, b
+        for _ in range(reps):
+            kern.launch_temporal([a], [b], self.halo, stream,
                                         mode=1 if (self.mode == "recompute" and not wrap) else 0,
                                         validate=False, periodic_axes=wrap)
             if not wrap and self.policy.mode != "static":   # a Dirichlet halo is never written: it stays valid
@@ -145,11 +201,12 @@ class TimeStepper:
             a, b = b, a
         return a, b
 
-    def _block_wrap(self) -> tuple:
+    def _block_wrap(self, u=None) -> tuple:
         """Periodic axes a temporal block reads around (no halo fills between
         blocks): all of them for a periodic problem on the register/LDS
         kernel, which wraps any distance."""
-        if self.policy.mode == "wrap" and "tblock" in self._block.variants:
+        kern = self._block if u is None else self.plan_for(u)[0]
+        if self.policy.mode == "wrap" and (kern is None or "tblock" in kern.variants):
             return tuple(range(self.ndim))
         return ()
 
@@ -164,7 +221,7 @@ class TimeStepper:
             self.fill(b, stream)          # constant halo: valid in both buffers from here on
         stale = False                     # the halo of `a` is out of date (wrapped launches)
         if self._block is not None:
-            bw = self._block_wrap()
+            bw = self._block_wrap(a)
             nblk = steps // self.T
             for _ in range(nblk):
                 a, b = self._blk(a, b, stream, bw)
@@ -223,10 +280,12 @@ class TimeStepper:
     def _march_graph(self, u, b, steps, stream, graph_steps):
         import torch
 
-        # warm the JIT outside the capture
+        # warm the JIT (and the depth choice) outside the capture
         self._single._function(self._single.pick([u], [b], self.halo))
         if self._block is not None:
-            self._block._function(self._block.temporal_key([u], [b], self.halo))
+            kern = self.plan_for(u)[0]
+            if kern is not None:
+                kern._function(kern.temporal_key([u], [b], self.halo))
         chunk = graph_steps or steps
         per = chunk if self._block is None else (chunk // self.T) * self.T or chunk
         reps, rest = divmod(steps, per)

@@ -128,6 +128,22 @@ SOLO = {                      # (kind, storage) -> (smallest side, steps per lau
 }
 
 
+# Candidates the solo threshold is measured over (``_Class._measure_solo``):
+# the depths of the temporal kernel and the sides of a job of the class.
+SOLO_DEPTHS = {"diffusion3d": (2, 4), "heat2d": (2, 4), "grayscott": (2, 4)}
+SOLO_SIDES = {"diffusion3d": (128, 160, 176, 192, 224, 256, 320, 384),
+              "heat2d": (768, 1024, 1280, 1536, 2048, 2800, 4096),
+              "grayscott": (1024, 1536, 2048, 2800, 4096)}
+
+
+def class_sides(lo: int, hi: int, count: int = 24) -> list:
+    """Log-spaced sides in [lo, hi]: the batched jobs of a class whose tile
+    buckets are tuned."""
+    if hi <= lo:
+        return [lo]
+    return sorted({int(round(lo * (hi / lo) ** (k / (count - 1)))) for k in range(count)})
+
+
 @dataclass
 class _Solo:
     """Kernels of a class's solo jobs: one step, and a block of ``T`` steps
@@ -141,17 +157,23 @@ class _Solo:
     wrap: tuple
 
 
-def warmup_jobs() -> List[Job]:
+def warmup_jobs(server: "Optional[StencilServer]" = None) -> List[Job]:
     """Jobs covering every class's kernels (a two-step small job of every
     (kind, storage) class and a solo-sized job per solo kind and storage,
     one temporal launch and one sweep), run before a timed stream so that no
-    kernel is built while the clock runs."""
+    kernel is built while the clock runs.  With ``server`` the solo sides
+    and depths are that server's classes' (measured) ones."""
     out = []
     for kind in KINDS:
         for dtype in ("float32", "float64"):
             out.append(Job(len(out), kind, dtype, 40 if kind == "diffusion3d" else 128, 2))
-    for (kind, dtype), (side, depth) in SOLO.items():
-        out.append(Job(len(out), kind, dtype, side, depth + 1))
+    for kind, dtype in SOLO:
+        side, depth = SOLO[(kind, dtype)]
+        if server is not None:
+            cls = server._class(Job(-1, kind, dtype, 1, 1))
+            side, depth = cls.solo_min, cls.depth
+        if side:
+            out.append(Job(len(out), kind, dtype, side, depth + 1))
     return out
 
 
@@ -173,11 +195,24 @@ class _Class:
             fn = list(w.gray_scott(dtype))
         probe = BatchedStencil(fn)
         vz = probe.vz
+        self.fn = fn
+        self.solo_min, self.depth = SOLO.get((kind, dtype), (None, 0))
+        tuned_solo = self._tuned_solo() if (kind, dtype) in SOLO else None
+        if tuned_solo is not None:
+            self.solo_min, self.depth = tuned_solo
+        # tile buckets of the batched kernel per size class: measured (the
+        # autotuner's cached choice for this stencil, storage, sizes and
+        # arch) where available, else these defaults
+        from cutile.runtime.autotune import tuned_buckets
+
+        top = self.solo_min - 1 if self.solo_min else (384 if kind == "diffusion3d" else 4096)
         if kind == "diffusion3d":
             buckets = [Bucket("rows", 8, 32 * vz, 16, 96), Bucket("rows", 8, 64 * vz, 32, None)]
+            tb = tuned_buckets(fn, class_sides(32, top), bounds=(96, None))
         else:
             buckets = None                       # the 2-D plane/rows buckets by row length
-        self.bs = BatchedStencil(fn, buckets=buckets)
+            tb = tuned_buckets(fn, class_sides(128, top))
+        self.bs = BatchedStencil(fn, buckets=tb or buckets)
         self.vz = vz
         self.ndim = 3 if kind == "diffusion3d" else 2
         self.halo = (1,) * (self.ndim - 1) + (vz,)   # contiguous-axis halo one vector wide
@@ -190,8 +225,15 @@ class _Class:
         self.tick = 0
         self.stream = None            # the class's own stream in the server
         self.solo = None
-        if (kind, dtype) in SOLO:
-            self.solo_min, depth = SOLO[(kind, dtype)]
+        if self.solo_min:
+            self.solo = self._solo_kernels(self.depth)
+
+    def _solo_kernels(self, depth: int) -> "_Solo":
+        """One-step and ``depth``-step kernels of this class's solo jobs."""
+        from cutile import workloads as w
+
+        fn, kind = self.fn, self.kind
+        if True:
             if kind == "grayscott":
                 from cutile import compile_fused
                 from cutile.runtime.launcher import compile_fused_block
@@ -199,19 +241,85 @@ class _Class:
                 # periodic: blocks and single steps read around the interior
                 one = compile_fused(fn)
                 blk = compile_fused_block(fn, depth, one, ["u", "v"])
-                self.solo = _Solo(one.load_module()._KERNEL, blk.load_module()._KERNEL, tuple(one.halo_widths),
-                                  depth, (0, 1))
-            else:
-                from cutile.runtime.timestep import TimeStepper
+                return _Solo(one.load_module()._KERNEL, blk.load_module()._KERNEL, tuple(one.halo_widths),
+                             depth, (0, 1))
+            from cutile.runtime.timestep import TimeStepper
 
-                # zero-Dirichlet blocks of T steps: the same states as T sweeps
-                st = TimeStepper(fn, w.dirichlet_spec(self.ndim), temporal_steps=depth)
-                if st.T != depth or st.block_reps != 1:
-                    raise RuntimeError(f"serving class {kind}: no one-launch temporal kernel of depth {depth}")
-                self.solo = _Solo(st._single, st._block, tuple(st.halo), depth, ())
+            # zero-Dirichlet blocks of T steps: the same states as T sweeps
+            ndim = 3 if kind == "diffusion3d" else 2
+            st = TimeStepper(fn, w.dirichlet_spec(ndim), temporal_steps=depth, split="fit")
+            if st.T != depth or st.block_reps != 1:
+                raise RuntimeError(f"serving class {kind}: no one-launch temporal kernel of depth {depth}")
+            return _Solo(st._single, st._block, tuple(st.halo), depth, ())
+
+    # -- measured solo threshold ---------------------------------------------------
+    def _tuned_solo(self):
+        """(smallest solo side, depth) from the autotuner's cached
+        measurement for this class (``tuned``), or None."""
+        from cutile.runtime.autotune import _arch, tuned
+
+        parts = {"what": "serving_solo", "kind": self.kind, "storage": self.dtype,
+                 "depths": list(SOLO_DEPTHS[self.kind]), "sides": list(SOLO_SIDES[self.kind])}
+        val = tuned(parts, self._measure_solo, _arch())
+        return None if not val else (int(val[0]), int(val[1]))
+
+    def _measure_solo(self):
+        """Per candidate side (``SOLO_SIDES``): time one job stepped by the
+        batched sweep alone, and stepped alone by each candidate depth's
+        temporal kernel; the solo side is the smallest candidate from which
+        the best depth wins at every larger candidate too."""
+        import torch
+
+        from cutile.runtime import rt
+        from cutile.runtime.batched import BatchedStencil
+        from cutile.runtime.kernel import default_stream, run_prepared
+
+        tdt = getattr(torch, self.dtype)
+        nd = 3 if self.kind == "diffusion3d" else 2
+        bs = BatchedStencil(self.fn)
+        s = default_stream()
+        nf = 2 if self.kind == "grayscott" else 1
+
+        def timed(fn, reps=8):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = rt.Event(), rt.Event()
+            e0.record(s)
+            for _ in range(reps):
+                fn()
+            e1.record(s)
+            return e0.elapsed_ms(e1) / reps
+
+        wins = []
+        for side in SOLO_SIDES[self.kind]:
+            cols = -(-side // bs.vz) * bs.vz
+            halo = (1,) * (nd - 1) + (bs.vz,)
+            shp = tuple(n + 2 * h for n, h in zip((side,) * (nd - 1) + (cols,), halo))
+            a = [torch.zeros(shp, dtype=tdt, device="cuda") for _ in range(nf)]
+            b = [torch.zeros(shp, dtype=tdt, device="cuda") for _ in range(nf)]
+            plan = bs.assemble([bs.record(a, b, halo, (side,) * nd)])
+            batched = timed(lambda: bs.launch(plan, s))
+            del a, b, plan
+            best = None
+            for depth in SOLO_DEPTHS[self.kind]:
+                sol = self._solo_kernels(depth)
+                shp = tuple(side + 2 * h for h in sol.halo)
+                a = [torch.zeros(shp, dtype=tdt, device="cuda") for _ in range(nf)]
+                b = [torch.zeros(shp, dtype=tdt, device="cuda") for _ in range(nf)]
+                rec = sol.block.prepare(a, b, sol.halo, temporal=True, mode=0, periodic_axes=sol.wrap)
+                ms = timed(lambda: run_prepared(rec, s)) / depth
+                del a, b, rec
+                if best is None or ms < best[0]:
+                    best = (ms, depth)
+            wins.append((side, best[0] < batched, best[1]))
+            torch.cuda.empty_cache()
+        for k, (side, _win, depth) in enumerate(wins):
+            if all(w for _, w, _ in wins[k:]):
+                return [side, depth]
+        return [0, 0]
 
     def is_solo(self, side: int) -> bool:
-        return self.solo is not None and side >= self.solo_min
+        return self.solo is not None and bool(self.solo_min) and side >= self.solo_min
 
     def shape(self, side: int) -> tuple:
         if self.is_solo(side):                           # the temporal kernel's own layout
@@ -573,7 +681,7 @@ def main(argv=None) -> int:
     jobs = generate_jobs(args.rate, args.seconds, args.seed)
     srv = StencilServer(hbm_budget=None if args.budget_gib is None else int(args.budget_gib * 2**30),
                         inflight=args.inflight)
-    srv.run(warmup_jobs(), realtime=False)          # build every class's kernels before the clock starts
+    srv.run(warmup_jobs(srv), realtime=False)       # build every class's kernels before the clock starts
     stats = srv.run(jobs, realtime=True,
                     log=lambda m: print(f"[serving] {m}", file=sys.stderr, flush=True))
     stats.update(rate=args.rate, seconds=args.seconds, seed=args.seed, offered_jobs=len(jobs))

@@ -25,7 +25,7 @@ def _march_vs_oracle(oracle, fn, n3, steps, T):
     from cutile.runtime.timestep import TimeStepper
 
     spec = w.periodic_spec(len(n3))
-    st = TimeStepper(fn, spec, temporal_steps=T)
+    st = TimeStepper(fn, spec, temporal_steps=T, split="fit")
     h = st.halo
     inter = np.random.default_rng(11).random(n3)
     u = st.empty_like_interior(n3)

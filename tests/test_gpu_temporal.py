@@ -65,7 +65,7 @@ def test_periodic_recompute_march(oracle, which, T, steps, shape):
 
     fn = getattr(w, which)()
     spec = w.periodic_spec(3)
-    ts = TimeStepper(fn, spec, temporal_steps=T)
+    ts = TimeStepper(fn, spec, temporal_steps=T, split="fit")
     assert ts.T == T and ts.mode == "recompute"
     halo = ts.halo
     print(f"{which} T={T} halo {halo} steps {steps}", flush=True)
@@ -82,7 +82,7 @@ def test_dirichlet_zero_blocks_match_march(oracle):
 
     fn = w.heat2d()
     spec = w.dirichlet_spec(2)
-    ts = TimeStepper(fn, spec, temporal_steps=4)
+    ts = TimeStepper(fn, spec, temporal_steps=4, split="fit")
     assert ts.mode == "zero" and ts.T == 4
     u0 = w.heat2d_init(100, ts.halo, seed=0)
     got = to_host(ts.march(to_dev(u0), 11))
@@ -97,7 +97,7 @@ def test_graph_march_matches_eager(oracle, T):
 
     fn = w.diffusion7()
     spec = w.periodic_spec(3)
-    ts = TimeStepper(fn, spec, temporal_steps=T)
+    ts = TimeStepper(fn, spec, temporal_steps=T, split="fit")
     u0 = w.sinusoid_init((16, 20, 24), ts.halo, seed=5)
     oracle.apply_boundary(u0, spec, ts.halo)
     eager = to_host(ts.march(to_dev(u0), 10))
@@ -136,7 +136,7 @@ def test_nonzero_dirichlet_with_a_temporal_depth_matches_march(oracle):
 
     fn = w.heat2d()
     spec = w.dirichlet_spec(2, value=1.5)
-    ts = TimeStepper(fn, spec, temporal_steps=4)
+    ts = TimeStepper(fn, spec, temporal_steps=4, split="fit")
     u0 = w.heat2d_init(64, ts.halo, seed=3)
     oracle.apply_boundary(u0, spec, ts.halo)
     got = to_host(ts.march(to_dev(u0), 7))
@@ -152,7 +152,7 @@ def test_split_deep_block_matches_march(oracle):
 
     fn = w.star25()
     spec = w.periodic_spec(3)
-    ts = TimeStepper(fn, spec, temporal_steps=4)
+    ts = TimeStepper(fn, spec, temporal_steps=4, split="fit")
     assert ts.block_reps == 2
     print(f"star25 depth 4 as 2 x 2, halo {ts.halo}", flush=True)
     u0 = w.sinusoid_init((20, 24, 40), ts.halo, seed=4)

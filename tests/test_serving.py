@@ -58,21 +58,23 @@ def test_float64_jobs_match_the_oracle(oracle):
     import torch
 
     from cutile import workloads as w
-    from cutile.serving import SOLO, Job, StencilServer, _Class
+    from cutile.serving import KINDS, SOLO, Job, StencilServer
 
     print("serving: float64 jobs vs oracle", flush=True)
     # jobs 3-5 are stepped alone: depth-2 launches (two, then one sweep for
     # the fifth step) and depth-4 launches (one, then two sweeps; job 5
     # through the coupled-field kernel)
+    srv = StencilServer(keep_results=True)
+    # the classes' solo thresholds (measured and cached, or the defaults)
+    solo = {k: srv._class(Job(-1, k, "float64", 1, 1)).solo_min or SOLO[k, "float64"][0] for k in KINDS}
     jobs = [Job(0, "heat2d", "float64", 90, 11), Job(1, "diffusion3d", "float64", 26, 7),
             Job(2, "grayscott", "float64", 61, 9),
-            Job(3, "diffusion3d", "float64", SOLO["diffusion3d", "float64"][0], 5),
-            Job(4, "heat2d", "float64", SOLO["heat2d", "float64"][0], 6),
-            Job(5, "grayscott", "float64", SOLO["grayscott", "float64"][0], 6)]
-    srv = StencilServer(keep_results=True)
+            Job(3, "diffusion3d", "float64", solo["diffusion3d"], 5),
+            Job(4, "heat2d", "float64", solo["heat2d"], 6),
+            Job(5, "grayscott", "float64", solo["grayscott"], 6)]
     srv.run([Job(j.id, j.kind, j.dtype, j.side, j.steps) for j in jobs], realtime=False)
     for j in jobs:
-        cls = _Class(j.kind, j.dtype)
+        cls = srv._class(j)
         cls.allocate(j)                              # the same seeded initial fields
         assert j.solo == (j.id >= 3)
         n, halo = j.side, cls.halo_of(j)
