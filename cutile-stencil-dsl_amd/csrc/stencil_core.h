@@ -314,6 +314,39 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
     return make_rsrc(reinterpret_cast<const S*>(a.in[f]) + lay.base(a.h0 + px), lay.bytes(a.h0 + px, ES));
   };
 
+  // In-plane byte offsets do not depend on the plane: they are computed
+  // once per workgroup (BUF_OOB where the access must not happen), so a
+  // plane step spends no VALU on the address arithmetic of the ring.
+  // qoff: my cells (queue loads); roff: the halo-ring (or whole-tile)
+  // vectors this thread fetches for every LDS buffer.
+  int qoff[KY];
+#pragma unroll
+  for (int k = 0; k < KY; ++k) {
+    const int y = y0 + ty + k * NTY;
+    const int yq = w1 ? wrapi(y, a.n1) : y;
+    const bool ok = zload && a.h1 + yq < s1 && ty + k * NTY < TY;
+    qoff[k] = ok ? lay.ioff(a.h1 + yq, a.h2 + zq) * ES : BUF_OOB;
+  }
+  int roff[NB > 0 ? NB : 1][NVB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int j = 0; j < NVB; ++j) {
+      const int i = tid + j * NT;
+      bool ok = (j < T::nvb(b)) && (i < T::gvecs(b));
+      int rr = 0, c = 0;
+      T::gvec_rc(b, ok ? i : 0, rr, c);
+      int gy = y0 - P::BRY[b] + rr;           // interior coordinates
+      int gz = z0 - T::rzv(b) + c * VZ;
+      if (w1) gy = wrapi(gy, a.n1);
+      if (w2) gz = wrapi(gz, a.n2);
+      gy += a.h1;
+      gz += a.h2;
+      ok = ok && gy >= 0 && gy < s1 && gz >= 0 && gz + VZ <= s2;
+      roff[b][j] = ok ? lay.ioff(gy, gz) * ES : BUF_OOB;
+    }
+  }
+
   // my cells of plane x of field f (cells outside the interior but inside
   // the array are loaded too: they are the halo other cells read)
   // (live = false issues the same loads out of range: zeros, no traffic)
@@ -321,11 +354,8 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
     const Rsrc r = in_rsrc(f, x);
 #pragma unroll
     for (int k = 0; k < KY; ++k) {
-      const int y = y0 + ty + k * NTY;
-      const int yq = w1 ? wrapi(y, a.n1) : y;
-      const bool ok = live && zload && a.h1 + yq < s1 && ty + k * NTY < TY;
       // a queue plane is read once per chunk: optionally non-temporal (P::NTL)
-      const Vec<S, VZ> w = bload<S, VZ, P::NTL ? BUF_NT : 0>(r, ok ? lay.ioff(a.h1 + yq, a.h2 + zq) * ES : BUF_OOB);
+      const Vec<S, VZ> w = bload<S, VZ, P::NTL ? BUF_NT : 0>(r, live ? qoff[k] : BUF_OOB);
 #pragma unroll
       for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
     }
@@ -338,18 +368,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
       const Rsrc r = in_rsrc(P::BF[b], x + P::BDX[b]);
 #pragma unroll
       for (int j = 0; j < NVB; ++j) {
-        const int i = tid + j * NT;
-        bool ok = live && (j < T::nvb(b)) && (i < T::gvecs(b));
-        int rr = 0, c = 0;
-        T::gvec_rc(b, ok ? i : 0, rr, c);
-        int gy = y0 - P::BRY[b] + rr;           // interior coordinates
-        int gz = z0 - T::rzv(b) + c * VZ;
-        if (w1) gy = wrapi(gy, a.n1);
-        if (w2) gz = wrapi(gz, a.n2);
-        gy += a.h1;
-        gz += a.h2;
-        ok = ok && gy >= 0 && gy < s1 && gz >= 0 && gz + VZ <= s2;
-        const Vec<S, VZ> w = bload<S, VZ>(r, ok ? lay.ioff(gy, gz) * ES : BUF_OOB);
+        const Vec<S, VZ> w = bload<S, VZ>(r, live ? roff[b][j] : BUF_OOB);
 #pragma unroll
         for (int v = 0; v < VZ; ++v) pb[b][j][v] = Cvt<S, C>::ld(w.v[v]);
       }
@@ -620,8 +639,13 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   }
 
   // plane x of input field f: its index in the array, and whether it lies there
+  // (the wrap is a compare-and-add loop: x is within a few interiors)
   auto in_plane = [&](int f, int x, bool& ok) -> Rsrc {
-    const int px = w0 ? wrapm(x, a.n0) : x;
+    int px = x;
+    if (w0) {
+      while (px < 0) px += a.n0;
+      while (px >= a.n0) px -= a.n0;
+    }
     ok = a.h0 + px >= 0 && a.h0 + px < a.s0;
     return make_rsrc(reinterpret_cast<const S*>(a.in[f]) + (i64)(a.h0 + (ok ? px : xb)) * plane, pbytes);
   };

@@ -227,7 +227,9 @@ def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
                     continue
                 cfg = TBlockConfig(ty, tz, vz, nt, gy, gz, T)
                 est = tblock_estimate(meta, cfg, storage, compute)
-                if est["lds"] > budget or est["vgpr"] > 0.85 * _vgpr_limit(nt):
+                # a forced shape skips the register estimate (the code
+                # object's actual registers are checked before any launch)
+                if est["lds"] > budget or (force is None and est["vgpr"] > 0.85 * _vgpr_limit(nt)):
                     continue
                 # lane-cells evaluated per output cell (ghost ring and idle
                 # lanes), with the penalties measured on MI355X for narrow
