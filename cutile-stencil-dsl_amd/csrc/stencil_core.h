@@ -146,6 +146,11 @@ template <class P> struct Tab {
     return p;
   }
   static constexpr int svec(int b) { return rzv(b) / P::VZ; }
+  // z-neighbours of my queue cells read across lanes (P::ZDPP): lane
+  // distances 1 .. ndz(b) on each side, for buffers copied from the queue
+  static constexpr bool zdpp(int b) { return P::ZDPP && fromq(b) && P::BRZ[b] > 0; }
+  static constexpr int ndz(int b) { return zdpp(b) ? (P::BRZ[b] + P::VZ - 1) / P::VZ : 0; }
+  static constexpr int ndz_max() { int m = 1; for (int k = 0; k < P::NB; ++k) m = ndz(k) > m ? ndz(k) : m; return m; }
   // the tile interior of buffer b is copied from the register queue (only
   // the halo ring is fetched from memory)
   static constexpr bool fromq(int b) {
@@ -189,16 +194,52 @@ template <class... P> constexpr int sweep_lds_bytes() {
   return m;
 }
 
+// Value of x in the lane DIR lanes away within the wave (DIR = +1: lane
+// l+1, DPP wave_shl:1; DIR = -1: lane l-1, wave_shr:1), 32 bits at a time.
+// Lanes whose source lies outside the wave get 0; callers never use those.
+template <int DIR, class C> ST_HD C lane_shift(C x) {
+  constexpr int ctrl = DIR > 0 ? 0x130 : 0x138;
+  if constexpr (sizeof(C) == 8) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = __builtin_amdgcn_update_dpp(0u, (unsigned)u, ctrl, 0xf, 0xf, false);
+    const unsigned hi = __builtin_amdgcn_update_dpp(0u, (unsigned)(u >> 32), ctrl, 0xf, 0xf, false);
+    return __builtin_bit_cast(C, ((unsigned long long)hi << 32) | lo);
+  } else {
+    static_assert(sizeof(C) == 4, "lane_shift: 4- or 8-byte values");
+    return __builtin_bit_cast(C, __builtin_amdgcn_update_dpp(0u, __builtin_bit_cast(unsigned, x), ctrl, 0xf, 0xf,
+                                                               false));
+  }
+}
+
 template <class P, int R, int KD = Tab<P>::KY> struct Cell {
   using C = typename P::C;
   using T = Tab<P>;
   const C (*q)[T::QN][KD][P::VZ];
   const C* lds;
   int k, v, ly, lz;
+  // P::ZDPP: my queue vectors' neighbours across lanes (zn[b][j]: lane
+  // +(j+1) for j < ndz(b), lane -(j-ndz(b)+1) after), my lane's column in
+  // the tile row and my lane in the wave
+  const C (*zn)[2 * T::ndz_max()][P::VZ] = nullptr;
+  int tzl = 0, wl = 0;
 
   template <int F, int DX, int DY, int DZ> ST_HD C g() const {
     if constexpr (DY == 0 && DZ == 0 && T::HASQ && DX >= P::QLO && DX <= P::QHI) {
       return q[F][(DX - P::QLO + R) % T::QN][k][v];
+    } else if constexpr (DY == 0 && DZ != 0 && P::ZDPP && T::HASQ && DX >= P::QLO && DX <= P::QHI &&
+                         P::QF[F] && T::buf(F, DX) >= 0 && T::zdpp(T::buf(F, DX))) {
+      // cell z+DZ: in my vector, else in the vector of the lane d lanes
+      // away (same tile row, same wave), else (row edge) the LDS halo
+      constexpr int b = T::buf(F, DX);
+      const int t = v + DZ;
+      if (t >= 0 && t < P::VZ) return q[F][(DX - P::QLO + R) % T::QN][k][t];
+      const int d = t >= 0 ? t / P::VZ : -((P::VZ - 1 - t) / P::VZ);
+      const int vv = t - d * P::VZ;
+      const int j = d > 0 ? d - 1 : T::ndz(b) - d - 1;
+      const bool lane_ok = (unsigned)(tzl + d) < (unsigned)T::NTZ && (unsigned)(wl + d) < 64u;
+      C val = zn[b][j][vv];
+      if (!lane_ok) val = lds[T::off(b) + (ly + P::BRY[b]) * T::pitch(b) + (lz + T::rzv(b) + DZ)];
+      return val;
     } else if constexpr (DY == 0 && DZ != 0 && DZ > -P::VZ && DZ < P::VZ && T::HASQ &&
                          DX >= P::QLO && DX <= P::QHI && P::QF[F]) {
       // a neighbour along z inside my own vector is in the queue as well (the
@@ -466,10 +507,32 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
       const int lyk = ty + k * NTY < TY ? ty + k * NTY : TY - 1;
       const int y = y0 + ty + k * NTY;
       const bool valid = ty + k * NTY < TY && y < a.n1 && zin;
+      // P::ZDPP: neighbour lanes' queue vectors (all lanes active here)
+      C zn[NB > 0 ? NB : 1][2 * T::ndz_max()][VZ];
+      if constexpr (P::ZDPP) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (!T::zdpp(b)) continue;
+          const int sl = (P::BDX[b] - P::QLO + R) % QN;
+          C up[VZ], dn[VZ];
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) up[v] = dn[v] = q[P::BF[b]][sl][k][v];
+#pragma unroll
+          for (int d = 0; d < T::ndz(b); ++d) {
+#pragma unroll
+            for (int v = 0; v < VZ; ++v) {
+              up[v] = lane_shift<+1>(up[v]);
+              dn[v] = lane_shift<-1>(dn[v]);
+              zn[b][d][v] = up[v];
+              zn[b][T::ndz(b) + d][v] = dn[v];
+            }
+          }
+        }
+      }
       C res[NOUT][VZ];
 #pragma unroll
       for (int v = 0; v < VZ; ++v) {
-        Cell<P, R> c{q, lds, k, v, lyk, tz * VZ + v};
+        Cell<P, R> c{q, lds, k, v, lyk, tz * VZ + v, zn, tz, tid & 63};
         C o[NO];
         P::eval(c, o);
         if constexpr (P::UPD == 0) {

@@ -247,7 +247,8 @@ def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
 
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,
                 nontemporal: bool = True, prefetch: int = 1, partial: bool = False,
-                ntload: bool = False, tblock: "TBlockConfig | None" = None, brick: tuple = (0, 0, 0)) -> str:
+                ntload: bool = False, tblock: "TBlockConfig | None" = None, brick: tuple = (0, 0, 0),
+                zdpp: bool = False) -> str:
     """Traits struct of one kernel variant.  ``partial``: rows may end in a
     partial vector (batched instances whose interior is not a multiple of
     the vector width).  ``tblock``: traits of a temporal-blocking variant
@@ -276,6 +277,7 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int NTS = {1 if nontemporal else 0}, PD = {int(prefetch)}, "
         f"PARTIAL = {1 if partial else 0}, NTL = {1 if ntload else 0};",
         f"  static constexpr int BR0 = {int(brick[0])}, BR1 = {int(brick[1])}, BR2 = {int(brick[2])};",
+        f"  static constexpr int ZDPP = {1 if zdpp and tblock is None and cfg.vz > 0 else 0};",
     ]
     if tblock is not None:
         lines.append(f"  static constexpr int TT = {tblock.T}, GY = {tblock.gy}, GZ = {tblock.gz};")
@@ -301,6 +303,11 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
     return "\n".join(lines)
 
 
+def sweep_tag(cfg: KernelConfig, zdpp: bool = False) -> str:
+    """Tag of a sweep variant; ``z`` marks z-neighbours read across lanes."""
+    return cfg.tag + ("z" if zdpp else "")
+
+
 def kernel_symbol(meta: StencilMeta, tag: str) -> str:
     return f"{meta.name}_sweep_{tag}"
 
@@ -320,7 +327,7 @@ def brick_symbol(meta: StencilMeta, tag: str, brick: tuple) -> str:
 def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage: str,
                     compute: str, temporal=None, nontemporal: bool = True, prefetch: int = 1,
                     ntload: bool = False, tblocks: Sequence[TBlockConfig] = (),
-                    bricked: Sequence[tuple] = ()) -> str:
+                    bricked: Sequence[tuple] = (), zdpp: bool = False) -> str:
     """Full HIP translation unit with one sweep kernel per variant (plus the
     temporal-blocking kernels: ``temporal`` = (config, T) of the LDS-ring
     kernel, ``tblocks`` the register/LDS ones)."""
@@ -331,8 +338,8 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
         "",
     ]
     for cfg in variants:
-        tag = cfg.tag
-        out.append(emit_traits(meta, cfg, storage, compute, tag, nontemporal, prefetch, ntload=ntload))
+        tag = sweep_tag(cfg, zdpp)
+        out.append(emit_traits(meta, cfg, storage, compute, tag, nontemporal, prefetch, ntload=ntload, zdpp=zdpp))
         out.append(
             f'extern "C" __global__ void __launch_bounds__({cfg.nt}) '
             f"{kernel_symbol(meta, tag)}(st::SweepArgs a) {{ st::sweep_entry<P_{tag}>(a); }}"

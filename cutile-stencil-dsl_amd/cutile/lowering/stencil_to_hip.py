@@ -27,7 +27,7 @@ from typing import Optional, Sequence
 from cutile.frontend.ir import find_func
 from cutile.lowering.emitter import CodeEmitter
 from cutile.lowering.hip_emitter import (ELEM_BYTES, KernelConfig, brick_symbol, emit_hip_source, kernel_symbol,
-                                        lds_bytes, plan_accesses, radius3, tblock_config, tblock_queue_range,
+                                        lds_bytes, plan_accesses, radius3, sweep_tag, tblock_config, tblock_queue_range,
                                         tblock_symbol, temporal_symbol, temporal_tile)
 from cutile.lowering.meta import StencilMeta, apply_update, extract_fused_meta, extract_meta
 
@@ -149,6 +149,9 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
         else:
             ty, tz = temporal_tile(meta.ndim, temporal_steps, radius3(meta), compute_t)
             temporal = (KernelConfig(ty=ty, tz=tz, vz=1, nt=256), int(temporal_steps))
+    # z-neighbours across lanes (DPP) instead of the LDS plane: an emitter
+    # option, off unless measured faster for the stencil
+    zdpp = bool((kernel_hints or {}).get("zdpp", 0))
     bricked = []
     if brick_size:
         bd = brick_dims(meta.ndim, brick_size)
@@ -157,13 +160,13 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
                           nontemporal=bool((kernel_hints or {}).get("nontemporal", 1)),
                           prefetch=int((kernel_hints or {}).get("prefetch", 1)),
                           ntload=bool((kernel_hints or {}).get("ntload", 0)), tblocks=tblocks,
-                          bricked=bricked)
+                          bricked=bricked, zdpp=zdpp)
     attr = _attrs(kernel_hints)
     if attr:
         src = src.replace('extern "C" __global__ void ', f'extern "C" __global__ void {attr}')
     variants = {
-        "vec": (kernel_symbol(meta, vec.tag), vec.ty, vec.tz, vec.vz, vec.nt),
-        "scalar": (kernel_symbol(meta, scal.tag), scal.ty, scal.tz, scal.vz, scal.nt),
+        "vec": (kernel_symbol(meta, sweep_tag(vec, zdpp)), vec.ty, vec.tz, vec.vz, vec.nt),
+        "scalar": (kernel_symbol(meta, sweep_tag(scal, zdpp)), scal.ty, scal.tz, scal.vz, scal.nt),
     }
     if temporal is not None:
         tcfg, T = temporal
