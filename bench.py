@@ -445,7 +445,12 @@ def batched_wave_c4(count=512, steps=200, seed=0, graph=True):
         return a
 
     # --- batched float32 --------------------------------------------------
-    bs = BatchedStencil(wave2d("float32"))
+    # tile buckets per size class: the autotuner's measured choice for these
+    # sides (the package's tuned table for this arch, or measured here)
+    from cutile.runtime.autotune import tuned_buckets
+
+    buckets = tuned_buckets(wave2d("float32"), sides)
+    bs = BatchedStencil(wave2d("float32"), buckets=buckets)
     h32 = (1, bs.vz)
     pairs, inter = [], []
     for s, c in zip(sides, centres):
@@ -523,7 +528,9 @@ def batched_wave_c4(count=512, steps=200, seed=0, graph=True):
     out = {"instances": count, "cells": cells, "steps": steps,
            "batched_fp32": {"ms_per_step": ms32 / steps, "gcell_updates_per_s": g32, "launches_per_step": 1},
            "plain_fp64": {"ms_per_step": ms64 / steps, "gcell_updates_per_s": g64v, "launches_per_step": count},
-           "speedup": g32 / g64v, "rel_err_fp32_vs_fp64": err}
+           "speedup": g32 / g64v, "rel_err_fp32_vs_fp64": err,
+           "buckets": [[b.mapping, b.ty, b.tz, b.xch, b.max_cols] for b in bs.buckets],
+           "buckets_from": "autotuner" if buckets is not None else "defaults"}
     log(f"batched wave: fp32 batched {g32:.1f} vs fp64 plain {g64v:.1f} Gcell-updates/s "
         f"= {g32 / g64v:.2f}x, rel err {err:.2e}")
     return out

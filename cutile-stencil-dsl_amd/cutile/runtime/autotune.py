@@ -34,6 +34,9 @@ from typing import Optional, Sequence
 
 _CACHE_VERSION = "hip-v1"
 
+# print every measured candidate (``python -m cutile.tune`` sets it)
+VERBOSE = False
+
 
 @dataclass
 class AutotuneResult:
@@ -349,8 +352,8 @@ def autotune_buckets(stencil_fn, sides: Sequence[int], storage: Optional[str] = 
                 ms = e0.elapsed_ms(e1) / iters
             except (ValueError, RuntimeError):
                 continue
-            if verbose:
-                print(f"  class <= {bound}: {b} {ms:.4f} ms")
+            if verbose or VERBOSE:
+                print(f"  class <= {bound}: {b} {ms:.4f} ms", flush=True)
             if best is None or ms < best[0]:
                 best = (ms, b)
         if best is None:
@@ -373,59 +376,76 @@ def split_candidates(T: int) -> list:
 def autotune_split(stepper, u, min_ms: float = 20.0, verbose: bool = False) -> int:
     """The depth d (a divisor of ``stepper.T``; 1 = single sweeps) whose
     launches advance arrays shaped like ``u`` fastest per step.  Each
-    candidate runs whole blocks of T steps on two scratch arrays of u's
-    shape (zeros, the stepper's boundary handling) for at least ``min_ms``;
-    depths whose levels fit neither registers nor LDS are skipped.  The
-    choice is cached per (stencil, storage, size bucket, T, mode, arch) in
-    the autotune cache.  Every candidate gives the same states."""
-    import torch
-
-    from cutile.runtime import rt
-    from cutile.runtime.kernel import current_arch, default_stream
+    candidate runs whole blocks of T steps, captured in a hipGraph and
+    replayed as step loops run, on two scratch arrays of u's shape (zeros,
+    the stepper's boundary handling) for about ``min_ms``; depths whose
+    levels fit neither registers nor LDS are skipped.  The choice is kept in
+    the tuned tables per (stencil, storage, size bucket, T, mode, arch).
+    Every candidate gives the same states."""
+    from cutile.frontend.ir import find_func
 
     T = stepper.T
     interior = tuple(int(n) - 2 * h for n, h in zip(u.shape, stepper.halo))
-    arch = current_arch().split(":")[0]
-    key = cache_key(stepper.fn, stepper.storage, interior, ("split", T, stepper.mode), arch)
-    hit = _load(key)
-    if hit is not None:
-        return int(hit.temporal_steps)
+    parts = {"what": "temporal_split", "stencil": stepper.fn._fn.__name__, "ir": _ir_digest(stepper.fn),
+             "storage": stepper.storage, "size": list(size_bucket(interior)), "T": T, "mode": stepper.mode,
+             "ndim": find_func(stepper.fn._ir).ndim}
+    val = tuned(parts, lambda: [_measure_split(stepper, u, min_ms, verbose)], _arch())
+    return int(val[0]) if val else T // stepper.block_reps
+
+
+def _measure_split(stepper, u, min_ms: float, verbose: bool) -> int:
+    import torch
+
+    from cutile.runtime import rt
+
+    T = stepper.T
     a = torch.zeros_like(u)
     b = torch.zeros_like(u)
-    s = default_stream()
     best = None
     saved = dict(stepper._plans)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
     for d in split_candidates(T):
         kern = None if d == 1 else stepper.depth_kernel(d)
         if d > 1 and kern is None:
             continue
         stepper._plans[(tuple(u.shape), u.dtype)] = (kern, T // d)
         try:
-            x, y = stepper.enqueue(a, b, T, s)              # build and warm up
-            torch.cuda.synchronize()
-            reps, ms = 0, 0.0
-            while ms < min_ms and reps < 64:
+            # the blocks are captured in a graph and replayed, as step loops
+            # run (eager timing would measure host launch overhead)
+            with torch.cuda.stream(side):
+                s = side.cuda_stream
+                stepper.enqueue(a, b, 2 * T, s)                 # build and warm up
+                rt.stream_sync(s)
                 e0, e1 = rt.Event(), rt.Event()
                 e0.record(s)
-                x, y = stepper.enqueue(x, y, T, s)
+                stepper.enqueue(a, b, 2 * T, s)
                 e1.record(s)
-                ms += e0.elapsed_ms(e1)
-                reps += 1
+                once = max(e0.elapsed_ms(e1), 1e-3)
+                blocks = 2 * max(1, min(32, int(min_ms / once)))
+                g = rt.Graph(s)
+                with g:
+                    stepper.enqueue(a, b, blocks * T, s)
+                g.launch(s)
+                rt.stream_sync(s)
+                e0, e1 = rt.Event(), rt.Event()
+                e0.record(s)
+                g.launch(s)
+                e1.record(s)
+                ms = e0.elapsed_ms(e1)
+                del g
         except (ValueError, RuntimeError):
             continue
-        per_step = ms / (reps * T)
-        if verbose:
-            print(f"  split T={T} as {T // d} x depth {d}: {per_step * 1e3:.2f} us/step")
+        per_step = ms / (blocks * T)
+        if verbose or VERBOSE:
+            print(f"  split T={T} as {T // d} x depth {d}: {per_step * 1e3:.2f} us/step", flush=True)
         if best is None or per_step < best[0]:
             best = (per_step, d)
     stepper._plans.clear()
     stepper._plans.update(saved)
     del a, b
-    d = best[1] if best is not None else T // stepper.block_reps
-    cells = math.prod(interior)
-    ms = best[0] if best is not None else 0.0
-    _save(key, AutotuneResult(tuple(interior), d, cells / (ms * 1e-3) / 1e9 if ms else 0.0, 0.0, ms, {}))
-    return d
+    torch.cuda.synchronize()
+    return best[1] if best is not None else T // stepper.block_reps
 
 
 # ---------------------------------------------------------------------------
@@ -486,15 +506,17 @@ def tuned(parts: dict, measure, arch: str):
 
 
 def export_tuned(arch: str, path: Optional[str] = None) -> str:
-    """Merge every tuned entry of the user cache into the shipped table for
-    ``arch`` (the package's ``tuned/<arch>.json`` unless ``path``)."""
+    """Merge the shipped table for ``arch`` (the package's
+    ``tuned/<arch>.json``), the file at ``path`` and every tuned entry of
+    the user cache into ``path`` (default: the shipped table)."""
     import glob
 
     path = path or tuned_path(arch)
     entries = {}
-    if os.path.exists(path):
-        for e in json.load(open(path))["entries"]:
-            entries[_parts_key(e["key"])] = e
+    for src in dict.fromkeys((tuned_path(arch), path)):     # the shipped table, then the target's
+        if os.path.exists(src):
+            for e in json.load(open(src))["entries"]:
+                entries[_parts_key(e["key"])] = e
     for f in glob.glob(os.path.join(cache_root(), "tuned-*.json")):
         e = json.load(open(f))
         if e["key"].get("arch") == arch:

@@ -258,16 +258,18 @@ class _Class:
         measurement for this class (``tuned``), or None."""
         from cutile.runtime.autotune import _arch, tuned
 
-        parts = {"what": "serving_solo", "kind": self.kind, "storage": self.dtype,
+        parts = {"what": "serving_solo_loaded", "kind": self.kind, "storage": self.dtype,
                  "depths": list(SOLO_DEPTHS[self.kind]), "sides": list(SOLO_SIDES[self.kind])}
         val = tuned(parts, self._measure_solo, _arch())
         return None if not val else (int(val[0]), int(val[1]))
 
     def _measure_solo(self):
-        """Per candidate side (``SOLO_SIDES``): time one job stepped by the
-        batched sweep alone, and stepped alone by each candidate depth's
-        temporal kernel; the solo side is the smallest candidate from which
-        the best depth wins at every larger candidate too."""
+        """Per candidate side (``SOLO_SIDES``): the time per job-step of a
+        batch of such jobs in one launch of the batched sweep (enough jobs
+        to fill the device, as the class runs under load), against one job
+        stepped alone by each candidate depth's temporal kernel; the solo
+        side is the smallest candidate from which the best depth wins at
+        every larger candidate too."""
         import torch
 
         from cutile.runtime import rt
@@ -295,11 +297,12 @@ class _Class:
             cols = -(-side // bs.vz) * bs.vz
             halo = (1,) * (nd - 1) + (bs.vz,)
             shp = tuple(n + 2 * h for n, h in zip((side,) * (nd - 1) + (cols,), halo))
-            a = [torch.zeros(shp, dtype=tdt, device="cuda") for _ in range(nf)]
-            b = [torch.zeros(shp, dtype=tdt, device="cuda") for _ in range(nf)]
-            plan = bs.assemble([bs.record(a, b, halo, (side,) * nd)])
-            batched = timed(lambda: bs.launch(plan, s))
-            del a, b, plan
+            jobs = max(1, min(16, -(-(32 << 20) // side ** nd)))      # ~32M cells per launch
+            bufs = [([torch.zeros(shp, dtype=tdt, device="cuda") for _ in range(nf)],
+                     [torch.zeros(shp, dtype=tdt, device="cuda") for _ in range(nf)]) for _ in range(jobs)]
+            plan = bs.assemble([bs.record(a, b, halo, (side,) * nd) for a, b in bufs])
+            batched = timed(lambda: bs.launch(plan, s)) / jobs
+            del bufs, plan
             best = None
             for depth in SOLO_DEPTHS[self.kind]:
                 sol = self._solo_kernels(depth)
@@ -312,6 +315,11 @@ class _Class:
                 if best is None or ms < best[0]:
                     best = (ms, depth)
             wins.append((side, best[0] < batched, best[1]))
+            from cutile.runtime import autotune as _at
+
+            if _at.VERBOSE:
+                print(f"  solo {self.kind} {self.dtype} side {side}: batched {batched * 1e3:.1f} us/step, "
+                      f"alone {best[0] * 1e3:.1f} us/step at depth {best[1]}", flush=True)
             torch.cuda.empty_cache()
         for k, (side, _win, depth) in enumerate(wins):
             if all(w for _, w, _ in wins[k:]):

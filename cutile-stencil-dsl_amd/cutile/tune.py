@@ -34,6 +34,9 @@ def main() -> None:
     from cutile.serving import KINDS, Job, StencilServer
 
     assert torch.cuda.is_available(), "tuning needs a device"
+    from cutile.runtime import autotune
+
+    autotune.VERBOSE = True
     t0 = time.time()
 
     def log(msg):
