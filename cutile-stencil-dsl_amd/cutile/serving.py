@@ -128,6 +128,14 @@ SOLO = {                      # (kind, storage) -> (smallest side, steps per lau
 }
 
 
+# Solo thresholds measured per class (``_Class._measure_solo``: one job alone
+# against the per-job time in a loaded batched launch) instead of ``SOLO``.
+# Off by default: on the 60-s stream (two runs each) the measured thresholds
+# gave p99 1.20 / 1.43 s against 1.32 / 1.00 s with ``SOLO``, which was set
+# from class streams under load; ``python -m cutile.tune --serving-solo``
+# turns it on.
+SOLO_TUNING = False
+
 # Candidates the solo threshold is measured over (``_Class._measure_solo``):
 # the depths of the temporal kernel and the sides of a job of the class.
 SOLO_DEPTHS = {"diffusion3d": (2, 4), "heat2d": (2, 4), "grayscott": (2, 4)}
@@ -197,7 +205,7 @@ class _Class:
         vz = probe.vz
         self.fn = fn
         self.solo_min, self.depth = SOLO.get((kind, dtype), (None, 0))
-        tuned_solo = self._tuned_solo() if (kind, dtype) in SOLO else None
+        tuned_solo = self._tuned_solo() if (kind, dtype) in SOLO and SOLO_TUNING else None
         if tuned_solo is not None:
             self.solo_min, self.depth = tuned_solo
         # tile buckets of the batched kernel per size class: measured (the

@@ -5,7 +5,8 @@ measurement the runtime looks up in its tuned tables:
 
 * the batched kernel's tile bucket per size class for the 512 wave
   instances (sides log-uniform in [64, 2048]) and for every serving class;
-* the serving classes' solo thresholds and temporal depths;
+* with ``--serving-solo``, the serving classes' solo thresholds and
+  temporal depths (otherwise the hand-set ``cutile.serving.SOLO`` table);
 * the depth split of temporal blocks (``TimeStepper(split="auto")``) for
   2-D heat at 1024^2 (T = 4, 10) and the 25-point and 7-point stars at
   512^3 (T = 2, 4, float64 and float32).
@@ -24,6 +25,8 @@ import time
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--export", help="write the merged tuned table for this device's arch here")
+    ap.add_argument("--serving-solo", action="store_true",
+                    help="also measure the serving classes' solo thresholds (used only with this flag)")
     args = ap.parse_args()
 
     import torch
@@ -31,6 +34,7 @@ def main() -> None:
     from cutile import workloads as w
     from cutile.runtime.autotune import _arch, export_tuned, tuned_buckets
     from cutile.runtime.timestep import TimeStepper
+    from cutile import serving
     from cutile.serving import KINDS, Job, StencilServer
 
     assert torch.cuda.is_available(), "tuning needs a device"
@@ -44,6 +48,7 @@ def main() -> None:
 
     b = tuned_buckets(w.wave2d("float32"), w.wave_sides(512, seed=0))
     log(f"wave2d float32, 512 instances: buckets {b}")
+    serving.SOLO_TUNING = bool(args.serving_solo)
     srv = StencilServer()
     for kind in KINDS:
         for dtype in ("float32", "float64"):
