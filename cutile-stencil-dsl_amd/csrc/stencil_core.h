@@ -821,17 +821,23 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     for_each_ic([&](auto Mc) {
       constexpr int L = TT - decltype(Mc)::value;     // level evaluated
       constexpr int RL = pmod(R - (L - 1) * D, QN);   // rotation of level L-1's queue
-      // level L is read only on the output tile widened by (TT-L) radii:
-      // a wave whose cells all lie outside skips the evaluation (its values
-      // there are never read, 0 is pushed)
+      // level L is read only on the output tile widened by (TT-L) radii.
+      // P::TBSKIP 1: a wave whose cells all lie outside skips the evaluation
+      // (its values there are never read, 0 is pushed); 2: only levels whose
+      // band is narrower than the region test it; 0 (default): every level
+      // evaluates the whole region straight through.  On gfx950 the
+      // branch-free form is the faster one (1-10% in 3-D, up to 18% for
+      // ten 2-D levels): the per-wave test splits each plane step into
+      // small blocks the scheduler cannot interleave.
       constexpr int WY = (TT - L) * P::RY, WZ = (TT - L) * P::RZ;
+      constexpr bool SKIP = P::TBSKIP == 1 || (P::TBSKIP == 2 && (WY < GY || WZ < GZ));
       const int p = s - (L - 1) * D;                  // plane evaluated
       C res[NF][KC][VZ];
 #pragma unroll
       for (int k = 0; k < KC; ++k) {
         const bool need = act[k] && ly[k] >= GY - WY && ly[k] < GY + OTY + WY &&
                           lz[k] + VZ > GZ - WZ && lz[k] < GZ + OTZ + WZ;
-        if (__builtin_amdgcn_ballot_w64(need) != 0) {
+        if (!SKIP || __builtin_amdgcn_ballot_w64(need) != 0) {
 #pragma unroll
           for (int v = 0; v < VZ; ++v) {
             Cell<P, RL, KC> c{q[L - 1], lds + (L - 1) * LE, k, v, ly[k], lz[k] + v};

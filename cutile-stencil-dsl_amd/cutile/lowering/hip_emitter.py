@@ -248,7 +248,7 @@ def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,
                 nontemporal: bool = True, prefetch: int = 1, partial: bool = False,
                 ntload: bool = False, tblock: "TBlockConfig | None" = None, brick: tuple = (0, 0, 0),
-                zdpp: bool = False) -> str:
+                zdpp: bool = False, tbskip: int = 0) -> str:
     """Traits struct of one kernel variant.  ``partial``: rows may end in a
     partial vector (batched instances whose interior is not a multiple of
     the vector width).  ``tblock``: traits of a temporal-blocking variant
@@ -280,7 +280,8 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int ZDPP = {1 if zdpp and tblock is None and cfg.vz > 0 else 0};",
     ]
     if tblock is not None:
-        lines.append(f"  static constexpr int TT = {tblock.T}, GY = {tblock.gy}, GZ = {tblock.gz};")
+        lines.append(f"  static constexpr int TT = {tblock.T}, GY = {tblock.gy}, GZ = {tblock.gz}, "
+                     f"TBSKIP = {int(tbskip)};")
     upd = meta.update
     ne = meta.num_exprs
     if upd is None:
@@ -327,7 +328,7 @@ def brick_symbol(meta: StencilMeta, tag: str, brick: tuple) -> str:
 def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage: str,
                     compute: str, temporal=None, nontemporal: bool = True, prefetch: int = 1,
                     ntload: bool = False, tblocks: Sequence[TBlockConfig] = (),
-                    bricked: Sequence[tuple] = (), zdpp: bool = False) -> str:
+                    bricked: Sequence[tuple] = (), zdpp: bool = False, tbskip: int = 0) -> str:
     """Full HIP translation unit with one sweep kernel per variant (plus the
     temporal-blocking kernels: ``temporal`` = (config, T) of the LDS-ring
     kernel, ``tblocks`` the register/LDS ones)."""
@@ -364,7 +365,8 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
         out.append("")
     for tb in tblocks:
         tag = tb.tag
-        out.append(emit_traits(meta, tb.kernel_config(), storage, compute, tag, nontemporal, tblock=tb))
+        out.append(emit_traits(meta, tb.kernel_config(), storage, compute, tag, nontemporal, tblock=tb,
+                               tbskip=tbskip))
         out.append(
             f'extern "C" __global__ void __launch_bounds__({tb.nt}) '
             f"{tblock_symbol(meta, tb)}(st::SweepArgs a) {{ st::tblock_entry<P_{tag}>(a); }}"

@@ -1,6 +1,6 @@
 """Measure and store the autotuner's choices for the configured workloads.
 
-``python -m cutile.tune [--export PATH]`` runs, on the current device, every
+``python -m cutile.tune [--export PATH] [--fresh]`` runs, on the current device, every
 measurement the runtime looks up in its tuned tables:
 
 * the batched kernel's tile bucket per size class for the 512 wave
@@ -27,6 +27,8 @@ def main() -> None:
     ap.add_argument("--export", help="write the merged tuned table for this device's arch here")
     ap.add_argument("--serving-solo", action="store_true",
                     help="also measure the serving classes' solo thresholds (used only with this flag)")
+    ap.add_argument("--fresh", action="store_true",
+                    help="measure every choice again, ignoring the shipped table (after the kernels changed)")
     args = ap.parse_args()
 
     import torch
@@ -41,6 +43,7 @@ def main() -> None:
     from cutile.runtime import autotune
 
     autotune.VERBOSE = True
+    autotune.IGNORE_SHIPPED = bool(args.fresh)
     t0 = time.time()
 
     def log(msg):

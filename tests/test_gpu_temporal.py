@@ -28,19 +28,22 @@ def _chain_zero_halo(oracle, u, fn, ndim, halo, T):
     return cur
 
 
-@pytest.mark.parametrize("which,T", [("heat2d", 4), ("diffusion7", 2), ("diffusion7", 3), ("star25", 2)])
-def test_zero_mode_equals_zero_halo_chain(oracle, which, T):
+@pytest.mark.parametrize("which,T,skip", [("heat2d", 4, 0), ("diffusion7", 2, 0), ("diffusion7", 3, 0), ("star25", 2, 0),
+                                          # the per-wave level skip (option tbskip 1 and 2)
+                                          ("heat2d", 4, 1), ("diffusion7", 3, 1), ("star25", 2, 1),
+                                          ("heat2d", 4, 2), ("diffusion7", 3, 2)])
+def test_zero_mode_equals_zero_halo_chain(oracle, which, T, skip):
     import torch
 
     from cutile import compile
     from cutile import workloads as w
 
     fn = getattr(w, which)()
-    r = compile(fn, temporal_steps=T)
+    r = compile(fn, temporal_steps=T, kernel_hints={"tbskip": skip} if skip else None)
     v = r.load_module()._VARIANTS
     key = "tblock" if "tblock" in v else "temporal"
     assert r.temporal_steps == T and key in v
-    print(f"{which} T={T} variant {v[key]}", flush=True)
+    print(f"{which} T={T} skip {skip} variant {v[key]}", flush=True)
     rng = np.random.default_rng(T)
     base = {2: (70, 150), 3: (23, 19, 45)}[r.ndim]
     shape = tuple(b + 2 * h for b, h in zip(base, r.halo_widths))
