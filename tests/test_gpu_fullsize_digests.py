@@ -115,7 +115,7 @@ def _c3_setup(dtype, T):
 
 
 @pytest.mark.parametrize("dtype,T", [("float64", 1), ("float64", 2), ("float64", 4), ("float32", 1),
-                                     ("float32", 2), ("float32", 4)])
+                                     ("float32", 2), ("float32", 4), ("bfloat16_kahan", 1)])
 def test_gray_scott_4096_every_50_steps(dtype, T):
     import torch
 
