@@ -12,12 +12,12 @@ namespace st {
 
 // NumPy's maximum/minimum propagate NaN (elementwise.py:25-52 swaps the
 // builtins for np.maximum/np.minimum); fmax/fmin would drop it.
-template <class T> ST_HD inline T maxp(T a, T b) {
+template <class T> ST_HD T maxp(T a, T b) {
   return (a != a) ? a : ((b != b) ? b : (a >= b ? a : b));
 }
-template <class T> ST_HD inline T minp(T a, T b) {
+template <class T> ST_HD T minp(T a, T b) {
   return (a != a) ? a : ((b != b) ? b : (a <= b ? a : b));
 }
-template <class T> ST_HD inline T absp(T a) { return a < T(0) ? -a : (a == T(0) ? T(0) : a); }
+template <class T> ST_HD T absp(T a) { return a < T(0) ? -a : (a == T(0) ? T(0) : a); }
 
 }  // namespace st

@@ -648,7 +648,14 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   static_assert(OTY > 0 && OTZ > 0 && GZ % VZ == 0 && TZ % VZ == 0, "temporal tile geometry");
   static_assert(TT >= 2, "at least two levels");
 
-  __shared__ C lds[TT * LE];
+  // P::TBDB: two sets of LDS planes used by alternate steps, so a step's
+  // plane writes need no barrier after the previous step's reads (one
+  // barrier per step instead of two)
+  constexpr int NBUF = P::TBDB ? 2 : 1;
+  // steps unrolled per loop trip: a multiple of QN (queue rotation) and,
+  // with two plane sets, of 2
+  constexpr int UN = (P::TBDB && QN % 2) ? 2 * QN : QN;
+  __shared__ C lds[NBUF * TT * LE];
 
   const int tid = threadIdx.x;
   const int y0 = tile_y * OTY - GY, z0 = tile_z * OTZ - GZ;   // region origin, interior coordinates
@@ -767,12 +774,14 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
         for (int k = 0; k < KC; ++k)
 #pragma unroll
           for (int v = 0; v < VZ; ++v) q[l][f][i][k][v] = C(0);
-  for (int i = tid; i < TT * LE; i += NT) lds[i] = C(0);
+  for (int i = tid; i < NBUF * TT * LE; i += NT) lds[i] = C(0);
+  if constexpr (NBUF == 2) __syncthreads();      // the first step writes without a leading barrier
 
-  // one step; R = (s - s_first) % QN.  Plane p of level l sits in queue
-  // slot (p - QLO - s_first) mod QN.
+  // one step; (s - s_first) % UN = Rc, R = (s - s_first) % QN.  Plane p of
+  // level l sits in queue slot (p - QLO - s_first) mod QN.
   auto step = [&](auto Rc, int s) {
-    constexpr int R = decltype(Rc)::value;
+    constexpr int R = decltype(Rc)::value % QN;
+    C* const ldsb = lds + (NBUF == 2 ? (decltype(Rc)::value & 1) : 0) * TT * LE;
     // level-0 head: plane s + QHI
 #pragma unroll
     for (int f = 0; f < NF; ++f)
@@ -780,13 +789,13 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       for (int k = 0; k < KC; ++k)
 #pragma unroll
         for (int v = 0; v < VZ; ++v) q[0][f][pmod(R - 1, QN)][k][v] = pq[f][k][v];
-    __syncthreads();
+    if constexpr (NBUF == 1) __syncthreads();
     // LDS planes: level l's buffer b holds plane s - l*D + BDX[b] of level l
     for_each_ic([&](auto Lc) {
       constexpr int L = decltype(Lc)::value;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        C* base = lds + L * LE + T::off(b);
+        C* base = ldsb + L * LE + T::off(b);
         const int sl = pmod(R - L * D + P::BDX[b] - QLO, QN);
 #pragma unroll
         for (int k = 0; k < KC; ++k) {
@@ -840,7 +849,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
         if (!SKIP || __builtin_amdgcn_ballot_w64(need) != 0) {
 #pragma unroll
           for (int v = 0; v < VZ; ++v) {
-            Cell<P, RL, KC> c{q[L - 1], lds + (L - 1) * LE, k, v, ly[k], lz[k] + v};
+            Cell<P, RL, KC> c{q[L - 1], ldsb + (L - 1) * LE, k, v, ly[k], lz[k] + v};
             C o[NF];
             P::eval(c, o);
 #pragma unroll
@@ -882,10 +891,10 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     }, typename mkseq<TT>::type{});
   };
 
-  for (int s = s_first; s <= s_last; s += QN) {
+  for (int s = s_first; s <= s_last; s += UN) {
     for_each_ic([&](auto Rc) {
       if (s + decltype(Rc)::value <= s_last) step(Rc, s + decltype(Rc)::value);
-    }, typename mkseq<QN>::type{});
+    }, typename mkseq<UN>::type{});
   }
 }
 

@@ -13,6 +13,13 @@ pytestmark = pytest.mark.skipif(not rt.available() or regcheck.readelf_path() is
                                 reason="needs libstencil_rt.so (hiprtc) and llvm-readelf")
 
 
+def _fused_block(dtype, T):
+    from cutile.runtime.launcher import compile_fused_block
+
+    fns = list(w.gray_scott(dtype))
+    return compile_fused_block(fns, T, compile_fused(fns), ["u", "v"])
+
+
 def _cases():
     return [
         ("heat2d", lambda: compile(w.heat2d())),
@@ -33,6 +40,12 @@ def _cases():
         ("star25_f32_T2", lambda: compile(w.star25("float32"), temporal_steps=2)),
         ("star25_f32_T4", lambda: compile(w.star25("float32"), temporal_steps=4)),
         ("star25_f64_T4", lambda: compile(w.star25(), temporal_steps=4)),
+        ("diffusion7_f32_T2", lambda: compile(w.diffusion7("float32"), temporal_steps=2)),
+        ("heat2d_f32_T4", lambda: compile(w.heat2d("float32"), temporal_steps=4)),
+        ("heat2d_f32_T10", lambda: compile(w.heat2d("float32"), temporal_steps=10)),
+        ("gray_scott_f64_T2", lambda: _fused_block("float64", 2)),
+        ("gray_scott_f64_T4", lambda: _fused_block("float64", 4)),
+        ("gray_scott_f32_T4", lambda: _fused_block("float32", 4)),
         ("gray_scott_kahan_bf16", lambda: compile_fused(list(w.gray_scott_rates("float32")), update="kahan",
                                                         targets=["u", "v"], storage="bfloat16")),
         ("gray_scott_kahan_f32", lambda: compile_fused(list(w.gray_scott_rates("float32")), update="kahan",
