@@ -847,9 +847,32 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
         const bool need = act[k] && ly[k] >= GY - WY && ly[k] < GY + OTY + WY &&
                           lz[k] + VZ > GZ - WZ && lz[k] < GZ + OTZ + WZ;
         if (!SKIP || __builtin_amdgcn_ballot_w64(need) != 0) {
+          // P::ZDPP: z-neighbours of level L-1 from the neighbour lanes'
+          // queue vectors (the LDS planes still serve the lanes at row ends)
+          C zn[NB > 0 ? NB : 1][2 * T::ndz_max()][VZ];
+          if constexpr (P::ZDPP) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              if (!T::zdpp(b)) continue;
+              const int sl = (P::BDX[b] - QLO + RL) % QN;
+              C up[VZ], dn[VZ];
+#pragma unroll
+              for (int v = 0; v < VZ; ++v) up[v] = dn[v] = q[L - 1][P::BF[b]][sl][k][v];
+#pragma unroll
+              for (int d = 0; d < T::ndz(b); ++d) {
+#pragma unroll
+                for (int v = 0; v < VZ; ++v) {
+                  up[v] = lane_shift<+1>(up[v]);
+                  dn[v] = lane_shift<-1>(dn[v]);
+                  zn[b][d][v] = up[v];
+                  zn[b][T::ndz(b) + d][v] = dn[v];
+                }
+              }
+            }
+          }
 #pragma unroll
           for (int v = 0; v < VZ; ++v) {
-            Cell<P, RL, KC> c{q[L - 1], ldsb + (L - 1) * LE, k, v, ly[k], lz[k] + v};
+            Cell<P, RL, KC> c{q[L - 1], ldsb + (L - 1) * LE, k, v, ly[k], lz[k] + v, zn, lz[k] / VZ, tid & 63};
             C o[NF];
             P::eval(c, o);
 #pragma unroll

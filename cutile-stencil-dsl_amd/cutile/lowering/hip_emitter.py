@@ -277,7 +277,7 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int NTS = {1 if nontemporal else 0}, PD = {int(prefetch)}, "
         f"PARTIAL = {1 if partial else 0}, NTL = {1 if ntload else 0};",
         f"  static constexpr int BR0 = {int(brick[0])}, BR1 = {int(brick[1])}, BR2 = {int(brick[2])};",
-        f"  static constexpr int ZDPP = {1 if zdpp and tblock is None and cfg.vz > 0 else 0};",
+        f"  static constexpr int ZDPP = {1 if zdpp and cfg.vz > 0 else 0};",
     ]
     if tblock is not None:
         lines.append(f"  static constexpr int TT = {tblock.T}, GY = {tblock.gy}, GZ = {tblock.gz}, "
@@ -373,7 +373,7 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
         fits = 2 * tblock_estimate(meta, tb, storage, compute)["lds"] <= 160 * 1024
         db = int(fits and (bool(tbdb) if tbdb is not None else tb.vz > 1))
         out.append(emit_traits(meta, tb.kernel_config(), storage, compute, tag, nontemporal, tblock=tb,
-                               tbskip=tbskip, tbdb=db))
+                               zdpp=zdpp, tbskip=tbskip, tbdb=db))
         out.append(
             f'extern "C" __global__ void __launch_bounds__({tb.nt}) '
             f"{tblock_symbol(meta, tb)}(st::SweepArgs a) {{ st::tblock_entry<P_{tag}>(a); }}"

@@ -1,8 +1,8 @@
-"""Sweeps whose z-neighbours are read across lanes (DPP wave shifts,
-``kernel_hints={"zdpp": 1}``) instead of from the LDS plane: bit-exact
-against the oracle on ragged shapes (tile-row and wave edges fall back to
-the LDS halo), on random multi-field stencils, and on wrapped periodic
-steps, in float64 and float32."""
+"""Sweeps and temporal levels whose z-neighbours are read across lanes (DPP
+wave shifts, ``kernel_hints={"zdpp": 1}``) instead of from the LDS plane:
+bit-exact against the oracle on ragged shapes (tile-row and wave edges fall
+back to the LDS halo), on random multi-field stencils, on wrapped periodic
+steps, in float64 and float32, and for depth-T temporal launches."""
 
 import numpy as np
 import pytest
@@ -77,3 +77,31 @@ def test_zdpp_wrapped_steps_match_oracle(oracle):
         oracle.apply_boundary(ref, spec, h)
     I = oracle.interior(shape, h)
     assert np.array_equal(to_host(a)[I], ref[I])
+
+
+@pytest.mark.parametrize("which,T,n", [("star25", 2, (23, 19, 45)), ("diffusion7", 3, (23, 19, 45)),
+                                       ("heat2d", 4, (70, 150)), ("heat2d", 3, (33, 47))])
+def test_zdpp_temporal_levels_match_oracle(oracle, which, T, n):
+    """The temporal kernel's levels with z-neighbours read across lanes
+    (level L-1's queue vectors of the neighbour lanes; region-row and wave
+    edges fall back to the level's LDS plane): one depth-T launch equals T
+    chained zero-halo sweeps, bit for bit."""
+    import torch
+
+    from cutile import compile
+    from cutile import workloads as w
+    from test_gpu_temporal import _chain_zero_halo
+
+    fn = getattr(w, which)()
+    r = compile(fn, temporal_steps=T, kernel_hints={"zdpp": 1})
+    v = r.load_module()._VARIANTS
+    assert "tblock" in v and "ZDPP = 1" in r.hip_source
+    print(f"zdpp temporal {which} T={T} {n} variant {v['tblock']}", flush=True)
+    shape = tuple(b + 2 * h for b, h in zip(n, r.halo_widths))
+    u = np.random.default_rng(T + len(n)).random(shape)
+    du = to_dev(u)
+    out = torch.zeros_like(du)
+    r.launcher(du, out)
+    ref = _chain_zero_halo(oracle, u, fn._fn, r.ndim, r.halo_widths, T)
+    inner = oracle.interior(shape, r.halo_widths)
+    assert np.array_equal(to_host(out)[inner], ref[inner])
