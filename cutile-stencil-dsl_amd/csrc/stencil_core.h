@@ -624,6 +624,8 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
 //            around, so intermediate levels there are exact periodic images
 //            (no halo is read or written along that axis).
 constexpr int pmod(int a, int n) { return ((a % n) + n) % n; }
+constexpr int gcd_i(int a, int b) { return b == 0 ? a : gcd_i(b, a % b); }
+constexpr int lcm_i(int a, int b) { return a / gcd_i(a, b) * b; }
 
 // index along a periodic axis of interior extent n (any distance)
 ST_HD int wrapm(int i, int n) {
@@ -652,9 +654,12 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   // plane writes need no barrier after the previous step's reads (one
   // barrier per step instead of two)
   constexpr int NBUF = P::TBDB ? 2 : 1;
-  // steps unrolled per loop trip: a multiple of QN (queue rotation) and,
-  // with two plane sets, of 2
-  constexpr int UN = (P::TBDB && QN % 2) ? 2 * QN : QN;
+  // P::TBPD: level-0 planes and rings are loaded PD steps ahead (1..4;
+  // 2 by default: 1-5 % faster than 1 on MI355X, 3 and 4 no better)
+  constexpr int PD = P::TBPD < 1 ? 1 : (P::TBPD > 4 ? 4 : P::TBPD);
+  // steps unrolled per loop trip: a multiple of QN (queue rotation), of the
+  // plane sets and of the prefetch stages
+  constexpr int UN = lcm_i(QN, lcm_i(NBUF, PD));
   __shared__ C lds[NBUF * TT * LE];
 
   const int tid = threadIdx.x;
@@ -721,8 +726,8 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   };
 
   C q[TT][NF][QN][KC][VZ];       // level l's queue of every field (level 0: the inputs)
-  C pq[NF][KC][VZ];              // level-0 planes prefetched for the next step
-  C pb[NB > 0 ? NB : 1][NVB][VZ];  // level-0 halo ring prefetched for the next step
+  C pq[PD][NF][KC][VZ];              // level-0 planes prefetched for the next PD steps
+  C pb[PD][NB > 0 ? NB : 1][NVB][VZ];  // level-0 halo rings prefetched for the next PD steps
 
   auto load_q0 = [&](int f, int x, C (&dst)[KC][VZ], bool live) {
     bool ok;
@@ -735,7 +740,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
     }
   };
-  auto load_ring = [&](int x, bool live) {
+  auto load_ring = [&](int x, bool live, C (&dst)[NB > 0 ? NB : 1][NVB][VZ]) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       bool ok;
@@ -745,7 +750,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
       for (int j = 0; j < NVB; ++j) {
         const Vec<S, VZ> w = bload<S, VZ>(r, ok ? roff[b][j] : BUF_OOB);
 #pragma unroll
-        for (int v = 0; v < VZ; ++v) pb[b][j][v] = Cvt<S, C>::ld(w.v[v]);
+        for (int v = 0; v < VZ; ++v) dst[b][j][v] = Cvt<S, C>::ld(w.v[v]);
       }
     }
   };
@@ -754,16 +759,19 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   const int s_last = xe - 1 + (TT - 1) * D;
 
   // ---- prologue: level-0 planes s_first+QLO .. s_first+QHI-1 in slots
-  //      0 .. QN-2, plane s_first+QHI and the ring of plane s_first
-  //      prefetched; levels >= 1 and the LDS start at 0 (the halo rings of
+  //      0 .. QN-2, planes s_first+QHI .. s_first+QHI+PD-1 and the rings of
+  //      planes s_first .. s_first+PD-1 prefetched (a chunk takes at least
+  //      TT steps); levels >= 1 and the LDS start at 0 (the halo rings of
   //      levels >= 1 are never written)
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
 #pragma unroll
     for (int i = 0; i + 1 < QN; ++i) load_q0(f, s_first + QLO + i, q[0][f][i], true);
-    load_q0(f, s_first + QHI, pq[f], true);
+#pragma unroll
+    for (int p = 0; p < PD; ++p) load_q0(f, s_first + p + QHI, pq[p][f], true);
   }
-  load_ring(s_first, true);
+#pragma unroll
+  for (int p = 0; p < PD; ++p) load_ring(s_first + p, true, pb[p]);
 #pragma unroll
   for (int l = 1; l < TT; ++l)
 #pragma unroll
@@ -781,6 +789,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   // level l sits in queue slot (p - QLO - s_first) mod QN.
   auto step = [&](auto Rc, int s) {
     constexpr int R = decltype(Rc)::value % QN;
+    constexpr int PI = decltype(Rc)::value % PD;     // prefetch stage of this step
     C* const ldsb = lds + (NBUF == 2 ? (decltype(Rc)::value & 1) : 0) * TT * LE;
     // level-0 head: plane s + QHI
 #pragma unroll
@@ -788,7 +797,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
 #pragma unroll
       for (int k = 0; k < KC; ++k)
 #pragma unroll
-        for (int v = 0; v < VZ; ++v) q[0][f][pmod(R - 1, QN)][k][v] = pq[f][k][v];
+        for (int v = 0; v < VZ; ++v) q[0][f][pmod(R - 1, QN)][k][v] = pq[PI][f][k][v];
     if constexpr (NBUF == 1) __syncthreads();
     // LDS planes: level l's buffer b holds plane s - l*D + BDX[b] of level l
     for_each_ic([&](auto Lc) {
@@ -814,7 +823,7 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
               T::gvec_rc(b, i, rr, c);
               C* d = base + rr * T::pitch(b) + c * VZ;
 #pragma unroll
-              for (int v = 0; v < VZ; ++v) d[v] = pb[b][j][v];
+              for (int v = 0; v < VZ; ++v) d[v] = pb[PI][b][j][v];
             }
           }
         }
@@ -823,8 +832,8 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
     __syncthreads();
     // prefetch the next step's level-0 plane and ring while this step computes
 #pragma unroll
-    for (int f = 0; f < NF; ++f) load_q0(f, s + 1 + QHI, pq[f], s + 1 <= s_last);
-    load_ring(s + 1, s + 1 <= s_last);
+    for (int f = 0; f < NF; ++f) load_q0(f, s + PD + QHI, pq[PI][f], s + PD <= s_last);
+    load_ring(s + PD, s + PD <= s_last, pb[PI]);
 
     // levels TT .. 1
     for_each_ic([&](auto Mc) {

@@ -248,7 +248,7 @@ def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,
                 nontemporal: bool = True, prefetch: int = 1, partial: bool = False,
                 ntload: bool = False, tblock: "TBlockConfig | None" = None, brick: tuple = (0, 0, 0),
-                zdpp: bool = False, tbskip: int = 0, tbdb: int = 0) -> str:
+                zdpp: bool = False, tbskip: int = 0, tbdb: int = 0, tbpd: int = 2) -> str:
     """Traits struct of one kernel variant.  ``partial``: rows may end in a
     partial vector (batched instances whose interior is not a multiple of
     the vector width).  ``tblock``: traits of a temporal-blocking variant
@@ -281,7 +281,7 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
     ]
     if tblock is not None:
         lines.append(f"  static constexpr int TT = {tblock.T}, GY = {tblock.gy}, GZ = {tblock.gz}, "
-                     f"TBSKIP = {int(tbskip)}, TBDB = {int(tbdb)};")
+                     f"TBSKIP = {int(tbskip)}, TBDB = {int(tbdb)}, TBPD = {int(tbpd)};")
     upd = meta.update
     ne = meta.num_exprs
     if upd is None:
@@ -329,7 +329,7 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
                     compute: str, temporal=None, nontemporal: bool = True, prefetch: int = 1,
                     ntload: bool = False, tblocks: Sequence[TBlockConfig] = (),
                     bricked: Sequence[tuple] = (), zdpp: bool = False, tbskip: int = 0,
-                    tbdb: "int | None" = None) -> str:
+                    tbdb: "int | None" = None, tbpd: int = 2) -> str:
     """Full HIP translation unit with one sweep kernel per variant (plus the
     temporal-blocking kernels: ``temporal`` = (config, T) of the LDS-ring
     kernel, ``tblocks`` the register/LDS ones)."""
@@ -369,11 +369,13 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
         # two sets of level planes (one barrier per step) where both fit the
         # LDS: 1-6 % faster on MI355X (tbdb 0/1 forces the choice).  Not for
         # the scalar variants (ragged rows): their queues already hold up to
-        # 250 registers, and the second set's addresses spill them
+        # 250 registers, and the second set's addresses (or a second
+        # prefetch stage) spill them
         fits = 2 * tblock_estimate(meta, tb, storage, compute)["lds"] <= 160 * 1024
         db = int(fits and (bool(tbdb) if tbdb is not None else tb.vz > 1))
+        # the deeper prefetch likewise only for the vector variants
         out.append(emit_traits(meta, tb.kernel_config(), storage, compute, tag, nontemporal, tblock=tb,
-                               zdpp=zdpp, tbskip=tbskip, tbdb=db))
+                               zdpp=zdpp, tbskip=tbskip, tbdb=db, tbpd=tbpd if tb.vz > 1 else 1))
         out.append(
             f'extern "C" __global__ void __launch_bounds__({tb.nt}) '
             f"{tblock_symbol(meta, tb)}(st::SweepArgs a) {{ st::tblock_entry<P_{tag}>(a); }}"

@@ -161,7 +161,7 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
                           prefetch=int((kernel_hints or {}).get("prefetch", 1)),
                           ntload=bool((kernel_hints or {}).get("ntload", 0)), tblocks=tblocks,
                           bricked=bricked, zdpp=zdpp, tbskip=int((kernel_hints or {}).get("tbskip", 0)),
-                          tbdb=(kernel_hints or {}).get("tbdb"))
+                          tbdb=(kernel_hints or {}).get("tbdb"), tbpd=int((kernel_hints or {}).get("tbpd", 2)))
     attr = _attrs(kernel_hints)
     if attr:
         src = src.replace('extern "C" __global__ void ', f'extern "C" __global__ void {attr}')
