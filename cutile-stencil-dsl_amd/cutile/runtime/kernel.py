@@ -12,6 +12,7 @@ the vector or scalar variant from the arrays' alignment, packs the by-value
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import threading
 from typing import Optional, Sequence
@@ -323,20 +324,26 @@ def sweep_chunk_length(n0: int, tiles: int, default: int, queue_depth: int) -> i
     return max(1, min((c, x) for t, c, x in cands if t <= 1.03 * tmin)[1])
 
 
-def tblock_chunk_length(n0: int, tiles: int, default: int, depth: int, slots: int) -> int:
+def tblock_chunk_length(n0: int, tiles: int, default: int, depth: int, slots: int, cus: int = 256) -> int:
     """Planes per workgroup of a temporal-blocking launch.
 
     ``slots`` workgroups run at once (CUs x workgroups per CU); a launch of
     ``tiles * c`` workgroups takes ``ceil(tiles * c / slots)`` rounds of
     ``ceil(n0 / c) + depth`` steps each (every chunk warms the skewed levels
-    up over ``depth`` steps).  The chunk count c minimising that product is
-    taken: a partly filled last round and short chunks both cost time (on
-    MI355X a 2.25-round launch ran 25 % slower than a 2-round one of the same
+    up over ``depth`` steps), and a step takes longer the more workgroups
+    share a CU (about as the square root of their number: 2-D heat at
+    1024^2, depth 4, ran 4.0 us/step with 2 workgroups a CU and 4.5 with
+    4).  The chunk count c minimising that product is taken: a partly
+    filled last round and short chunks both cost time (on MI355X a
+    2.25-round launch ran 25 % slower than a 2-round one of the same
     kernel).  Chunks stay within the compiled chunk length ``default``."""
+    cus = max(1, cus)
+    per_cu_max = max(1, slots // cus)
     best = None
     for c in range(max(1, -(-n0 // max(1, default))), n0 + 1):
         x = -(-n0 // c)
-        t = -(-(tiles * c) // max(1, slots)) * (x + depth)
+        shared = min(-(-(tiles * c) // cus), per_cu_max)
+        t = -(-(tiles * c) // max(1, slots)) * (x + depth) * math.sqrt(shared)
         if best is None or t < best[0]:
             best = (t, x)
         if tiles * c > 64 * max(1, slots):
@@ -495,7 +502,7 @@ class StencilKernel:
                 xch = self._tb_chunks.get(ck)
                 if xch is None:
                     slots = cu_count() * workgroups_per_cu(self.resources(key), nt)
-                    xch = tblock_chunk_length(n[0], a.ntz * a.nty, self.xch, self.tblock_depth, slots)
+                    xch = tblock_chunk_length(n[0], a.ntz * a.nty, self.xch, self.tblock_depth, slots, cu_count())
                     self._tb_chunks[ck] = xch
             x = xch
         else:

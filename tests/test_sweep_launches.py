@@ -122,6 +122,18 @@ def test_3d_sweep_chunks_fill_whole_rounds(n0, tiles, qd, wgs):
     assert tiles * -(-n0 // x) == wgs
 
 
+@pytest.mark.parametrize("n0,tiles,depth,slots,x", [(1024, 4, 9, 1280, 8), (1024, 4, 12, 1280, 16),
+                                                    (4096, 16, 9, 1280, 52), (512, 496, 9, 256, 512),
+                                                    (512, 176, 9, 256, 128)])
+def test_temporal_chunks_weigh_workgroups_sharing_a_cu(n0, tiles, depth, slots, x):
+    """Temporal chunks: whole rounds first, then warm-up steps against
+    workgroups sharing a CU.  2-D heat at 1024^2 (4 column tiles) takes
+    chunks of 8 or 16 rows (2 or 1 workgroups a CU) instead of 4 (4 a CU,
+    measured 10 % slower); the 512^3 launches, which fill whole rounds,
+    keep their chunks."""
+    assert K.tblock_chunk_length(n0, tiles, n0, depth, slots, 256) == x
+
+
 @pytest.mark.parametrize("dtype,n", [("float64", 1536), ("float64", 1537), ("float64", 4095), ("float32", 2800),
                                      ("float32", 2801), ("float32", 4095)])
 def test_serving_solo_gray_scott_steps_in_bounds(dtype, n):
