@@ -930,13 +930,453 @@ ST_HD void tblock_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// two time levels per launch, one level per wave
+// ---------------------------------------------------------------------------
+//
+// A workgroup owns an output tile of OTY x OTZ cells and a chunk of planes.
+// Its threads split by whole waves into two roles:
+//   * NA level-1 threads own the region (the output tile widened by GY rows
+//     and GZ columns per side), KA vector cells each.  They keep a register
+//     queue of QN level-0 planes of their cells, loaded from HBM, read the
+//     in-plane neighbours from an LDS copy of the level-0 plane (its interior
+//     written from their queues), and write level 1 into an LDS ring of NR
+//     planes;
+//   * the NT - NA level-2 threads own the output tile only, KB vector cells
+//     each.  They keep a register queue of QN level-1 planes of their cells,
+//     read from the ring, take the in-plane neighbours from the ring, store
+//     level 2, and fetch the level-0 halo ring from HBM.
+// Level 2 is thus evaluated on the output tile only (no lanes spent on the
+// ghost ring), and no wave holds the queues of both levels, so a workgroup
+// of 16 waves fits 4 waves per SIMD.
+//
+// Step s: the level-1 threads evaluate plane s, the level-2 threads plane
+// s - D (D = QHI + 1), whose queue reaches plane s - 1.  The ring holds
+// planes s - D .. s (NR = D + 1).  Level-0 planes alternate between two LDS
+// buffers: plane s is read while plane s + 1 is written.  One barrier per
+// step: ring slot s % NR and level-0 buffer (s + 1) % 2 were last read in
+// step s - 1.  In-plane accesses must all lie in the plane being evaluated
+// (dx = 0), so every LDS buffer is the plane's own.
+//
+// Values of level 1 outside the interior follow tblock_tile (tmode 0: 0;
+// tmode 1: evaluated; wrapped axes: periodic images).
+
+// Planes x, x + 1, ... of an array streamed one per step: the plane's
+// element offset is kept incrementally (one add per plane, and a compare
+// and subtract where a periodic stream axis wraps) instead of a wrap loop
+// and a 64-bit multiply per plane.
+struct PlaneWalk {
+  i64 off;    // (h0 + px) * plane
+  int px;     // plane index, wrapped into [0, n0) along a periodic stream axis
+  ST_HD void init(int x, const SweepArgs& a, bool w0, i64 plane) {
+    px = x;
+    if (w0) {
+      while (px < 0) px += a.n0;
+      while (px >= a.n0) px -= a.n0;
+    }
+    off = (i64)(a.h0 + px) * plane;
+  }
+  ST_HD void next(const SweepArgs& a, bool w0, i64 plane, i64 span) {
+    ++px;
+    off += plane;
+    if (w0 && px >= a.n0) {
+      px -= a.n0;
+      off -= span;
+    }
+  }
+  ST_HD bool ok(const SweepArgs& a) const { return a.h0 + px >= 0 && a.h0 + px < a.s0; }
+};
+
+// Layout of the level-1 ring: row-major over the region, each row holding
+// the NR planes (slots) one after the other, each plane the NF fields, each
+// field PR elements (>= TZ).  A row then spans NR * NF * PR elements, so an
+// access of the level-2 threads is a constant offset of at most about ten
+// rows from their base (LDS instruction offsets reach 64 KiB).
+// The level-2 threads read rows of OTZ / VZ lanes.  On gfx950 a 16-byte LDS
+// read of a wave runs without bank conflicts when, within each half-wave,
+// lane l's 16-byte slot (address / 16 mod 16) is l plus a constant
+// (measured: rows of 16 lanes 320 B apart spent half their LDS cycles in
+// conflicts, 256 B apart none).  So PR makes the row span congruent to the
+// output width modulo 256 bytes, unless an output row fills whole
+// half-waves.
+template <class P> constexpr int ring_pitch() {
+  constexpr int OTZ = P::TZ - 2 * P::GZ, NR = P::QHI + 2;
+  if ((OTZ / P::VZ) % 32 == 0) return P::TZ;
+  int p = P::TZ;
+  while (((NR * P::NF * p - OTZ) * (int)sizeof(typename P::C)) % 256 != 0) ++p;
+  return p;
+}
+
+// level-1 ring view handed to the expression: column accesses from the
+// register queue (rotation R), in-plane neighbours from ring plane SL.
+// `nb` points at the ring cell (row - RY, column - RZ) of my vector's first
+// cell in slot 0, so every in-plane access is a non-negative constant
+// offset from it (LDS instruction offsets are unsigned).
+template <class P, int R, int KD, int SL> struct RingCell {
+  using C = typename P::C;
+  using T = Tab<P>;
+  static constexpr int PR = ring_pitch<P>();             // one field of one plane in a row
+  static constexpr int SS = P::NF * PR;                  // one plane in a row
+  static constexpr int RP = (P::QHI + 2) * SS;           // one row
+  const C (*q)[T::QN][KD][P::VZ];
+  const C* nb;
+  int k, v;
+  template <int F, int DX, int DY, int DZ> ST_HD C g() const {
+    if constexpr (DY == 0 && DZ == 0) {
+      static_assert(DX >= P::QLO && DX <= P::QHI, "column access outside the queue");
+      return q[F][(DX - P::QLO + R) % T::QN][k][v];
+    } else {
+      static_assert(DX == 0, "in-plane accesses of the two-level kernel lie in the evaluated plane");
+      static_assert(DY >= -P::RY && DY <= P::RY && DZ >= -P::RZ && DZ <= P::RZ, "access beyond the radius");
+      if constexpr (DY == 0 && DZ > -P::VZ && DZ < P::VZ) {
+        // a z-neighbour inside my own vector is in the queue
+        if (v + DZ >= 0 && v + DZ < P::VZ) return q[F][(0 - P::QLO + R) % T::QN][k][v + DZ];
+      }
+      return nb[(DY + P::RY) * RP + SL * SS + F * PR + (v + DZ + P::RZ)];
+    }
+  }
+};
+
+template <class P>
+ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe) {
+  using S = typename P::S;
+  using C = typename P::C;
+  using T = Tab<P>;
+  constexpr int NT = P::NT, NA = P::NA, NBT = NT - NA, TY = P::TY, TZ = P::TZ, VZ = P::VZ, NB = P::NB;
+  constexpr int NF = P::NF, QLO = P::QLO, QHI = P::QHI, QN = T::QN, D = QHI + 1, NR = D + 1;
+  constexpr int GY = P::GY, GZ = P::GZ, OTY = TY - 2 * GY, OTZ = TZ - 2 * GZ;
+  constexpr int NTZ = TZ / VZ, CELLS = TY * NTZ, KA = (CELLS + NA - 1) / NA;
+  constexpr int ONTZ = OTZ / VZ, OCELLS = OTY * ONTZ, KB = (OCELLS + NBT - 1) / NBT;
+  constexpr int ES = (int)sizeof(S);
+  constexpr int LE = T::lds_elems();               // one level-0 plane (every buffer)
+  constexpr int PR = ring_pitch<P>(), SS = NF * PR, RP = NR * SS;   // ring: field, plane, row
+  constexpr int PD = P::TBPD < 1 ? 1 : (P::TBPD > 4 ? 4 : P::TBPD);
+  constexpr int UN = lcm_i(QN, lcm_i(2, lcm_i(NR, PD)));
+  static_assert(P::TT == 2 && P::NO == NF && P::UPD == 0, "two levels; result m is the next state of field m");
+  static_assert(NA % 64 == 0 && NBT > 0 && NBT % 64 == 0, "roles are whole waves");
+  static_assert(QLO <= -1 && QHI >= 1, "column accesses on both sides of the plane");
+  static_assert(OTY > 0 && OTZ > 0 && GZ % VZ == 0 && TZ % VZ == 0 && GY >= P::RY && GZ >= P::RZ,
+                "two-level tile geometry");
+  // every buffer is a plane's own (dx = 0) and the fields' buffers are distinct
+  constexpr int NVB = [] {
+    int m = 1;
+    for (int b = 0; b < NB; ++b) {
+      const int g = T::gvecs(b);
+      m = (g + NBT - 1) / NBT > m ? (g + NBT - 1) / NBT : m;
+    }
+    return m;
+  }();
+  static_assert([] {
+    for (int b = 0; b < NB; ++b)
+      if (P::BDX[b] != 0 || !T::fromq(b)) return false;
+    return true;
+  }(), "in-plane accesses at dx = 0 only");
+  __shared__ __attribute__((aligned(16))) C lds[2 * LE + TY * RP];
+  C* const ring = lds + 2 * LE;
+
+  const int tid = threadIdx.x;
+  const int y0 = tile_y * OTY - GY, z0 = tile_z * OTZ - GZ;   // region origin, interior coordinates
+  const int s2 = (int)a.s2, s1 = (int)a.s1;
+  const i64 plane = a.s1 * a.s2;
+  const int pbytes = (int)(plane * ES);
+  const bool w0 = a.wrap & 1, w1 = a.wrap & 2, w2 = a.wrap & 4;
+  const int s_first = xb + QLO, s_last = xe - 1 + D;
+  const i64 span = (i64)a.n0 * plane;            // one wrap of a periodic stream axis
+
+  // plane x of input field f (wrapped along a periodic stream axis)
+  auto in_plane = [&](int f, int x, bool& ok) -> Rsrc {
+    int px = x;
+    if (w0) {
+      while (px < 0) px += a.n0;
+      while (px >= a.n0) px -= a.n0;
+    }
+    ok = a.h0 + px >= 0 && a.h0 + px < a.s0;
+    return make_rsrc(reinterpret_cast<const S*>(a.in[f]) + (i64)(a.h0 + (ok ? px : xb)) * plane, pbytes);
+  };
+
+  for (int i = tid; i < 2 * LE + TY * RP; i += NT) lds[i] = C(0);
+  __syncthreads();
+
+  if (__builtin_amdgcn_readfirstlane(tid) < NA) {
+    // ---------------- level 1 ----------------
+    const int t = tid;
+    int ly[KA], lz[KA], qoff[KA];
+    bool act[KA], keep[KA];
+#pragma unroll
+    for (int k = 0; k < KA; ++k) {
+      const int c = t + k * NA;
+      act[k] = c < CELLS;
+      const int cc = act[k] ? c : CELLS - 1;
+      ly[k] = cc / NTZ;
+      lz[k] = (cc % NTZ) * VZ;
+      int y = y0 + ly[k], z = z0 + lz[k];
+      keep[k] = (w1 || (y >= 0 && y < a.n1)) && (w2 || (z >= 0 && z < a.n2));
+      if (w1) y = wrapm(y, a.n1);
+      if (w2) z = wrapm(z, a.n2);
+      const bool ok = act[k] && a.h1 + y >= 0 && a.h1 + y < s1 && a.h2 + z >= 0 && a.h2 + z + VZ <= s2;
+      qoff[k] = ok ? ((a.h1 + y) * s2 + a.h2 + z) * ES : BUF_OOB;
+    }
+    C q[NF][QN][KA][VZ];     // level-0 planes; plane p in slot (p - QLO - s_first) mod QN
+    C pq[PD][NF][KA][VZ];    // queue heads loaded PD steps ahead
+    auto load_q0 = [&](int f, int x, C (&dst)[KA][VZ], bool live) {
+      bool ok;
+      const Rsrc r = in_plane(f, x, ok);
+      ok = ok && live;
+#pragma unroll
+      for (int k = 0; k < KA; ++k) {
+        const Vec<S, VZ> w = bload<S, VZ>(r, ok ? qoff[k] : BUF_OOB);
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
+      }
+    };
+    // the queue heads loaded in the step loop: planes s_first + QHI + PD, ...
+    PlaneWalk wq;
+    wq.init(s_first + QHI + PD, a, w0, plane);
+    const S* xb_plane[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) xb_plane[f] = reinterpret_cast<const S*>(a.in[f]) + (i64)(a.h0 + xb) * plane;
+    auto load_q0w = [&](int f, C (&dst)[KA][VZ], bool live) {
+      const bool ok = live && wq.ok(a);
+      const Rsrc r = make_rsrc(ok ? reinterpret_cast<const S*>(a.in[f]) + wq.off : xb_plane[f], pbytes);
+#pragma unroll
+      for (int k = 0; k < KA; ++k) {
+        const Vec<S, VZ> w = bload<S, VZ>(r, ok ? qoff[k] : BUF_OOB);
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) dst[k][v] = Cvt<S, C>::ld(w.v[v]);
+      }
+    };
+    // the interior of level-0 plane (queue slot SL) into LDS plane set `dst`
+    auto put_l0 = [&](C* dst, int sl) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+#pragma unroll
+        for (int k = 0; k < KA; ++k) {
+          if (act[k]) {
+            C* d = dst + T::off(b) + (P::BRY[b] + ly[k]) * T::pitch(b) + T::rzv(b) + lz[k];
+#pragma unroll
+            for (int v = 0; v < VZ; ++v) d[v] = q[P::BF[b]][sl][k][v];
+          }
+        }
+      }
+    };
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+#pragma unroll
+      for (int i = 0; i + 1 < QN; ++i) load_q0(f, s_first + QLO + i, q[f][i], true);
+#pragma unroll
+      for (int p = 0; p < PD; ++p) load_q0(f, s_first + QHI + p, pq[p][f], true);
+    }
+    put_l0(lds, -QLO);                   // plane s_first
+    __syncthreads();
+
+    auto step = [&](auto Rc, int s) {
+      constexpr int r = decltype(Rc)::value;
+      constexpr int R = r % QN, PI = r % PD;
+      // queue head: plane s + QHI
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int k = 0; k < KA; ++k)
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) q[f][pmod(R - 1, QN)][k][v] = pq[PI][f][k][v];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) load_q0w(f, pq[PI][f], s + PD <= s_last);
+      wq.next(a, w0, plane, span);
+      // level 1 at plane s
+      const C* l0 = lds + (r & 1) * LE;
+      C* const dst = ring + (r % NR) * SS;
+      const bool zero_pl = a.tmode == 0 && !(w0 || (s >= 0 && s < a.n0));
+#pragma unroll
+      for (int k = 0; k < KA; ++k) {
+        C res[NF][VZ];
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) {
+          Cell<P, R, KA> c{q, l0, k, v, ly[k], lz[k] + v};
+          C o[NF];
+          P::eval(c, o);
+#pragma unroll
+          for (int m = 0; m < NF; ++m) res[m][v] = o[m];
+        }
+        const bool zero = a.tmode == 0 && (zero_pl || !keep[k]);
+        if (act[k]) {
+#pragma unroll
+          for (int m = 0; m < NF; ++m) {
+            C* d = dst + ly[k] * RP + m * PR + lz[k];
+#pragma unroll
+            for (int v = 0; v < VZ; ++v) d[v] = zero ? C(0) : res[m][v];
+          }
+        }
+      }
+      // level-0 plane s + 1 into the other buffer
+      put_l0(lds + ((r + 1) & 1) * LE, pmod(r + 1 - QLO, QN));
+      __syncthreads();
+    };
+    for (int s = s_first; s <= s_last; s += UN) {
+      for_each_ic([&](auto Rc) {
+        if (s + decltype(Rc)::value <= s_last) step(Rc, s + decltype(Rc)::value);
+      }, typename mkseq<UN>::type{});
+    }
+  } else {
+    // ---------------- level 2 ----------------
+    const int t = tid - NA;
+    int soff[KB];
+    const C* nb[KB];          // ring cell (row - RY, column - RZ) of my vector, plane slot 0
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int c = t + k * NBT;
+      const bool act = c < OCELLS;
+      const int cc = act ? c : OCELLS - 1;
+      const int ly = GY + cc / ONTZ, lz = GZ + (cc % ONTZ) * VZ;
+      nb[k] = ring + (ly - P::RY) * RP + (lz - P::RZ);
+      const int y = y0 + ly, z = z0 + lz;
+      const bool mine = act && y >= 0 && y < a.n1 && z >= 0 && z < a.n2;
+      soff[k] = mine ? ((a.h1 + y) * s2 + a.h2 + z) * ES : BUF_OOB;
+    }
+    // level-0 halo-ring vectors this thread fetches: in-plane offsets and
+    // their place in the LDS plane
+    int roff[NB > 0 ? NB : 1][NVB], loff[NB > 0 ? NB : 1][NVB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+#pragma unroll
+      for (int j = 0; j < NVB; ++j) {
+        const int i = t + j * NBT;
+        bool ok = i < T::gvecs(b);
+        int rr = 0, c = 0;
+        T::gvec_rc(b, ok ? i : 0, rr, c);
+        loff[b][j] = ok ? T::off(b) + rr * T::pitch(b) + c * VZ : -1;
+        int gy = y0 - P::BRY[b] + rr, gz = z0 - T::rzv(b) + c * VZ;
+        if (w1) gy = wrapm(gy, a.n1);
+        if (w2) gz = wrapm(gz, a.n2);
+        gy += a.h1;
+        gz += a.h2;
+        ok = ok && gy >= 0 && gy < s1 && gz >= 0 && gz + VZ <= s2;
+        roff[b][j] = ok ? (gy * s2 + gz) * ES : BUF_OOB;
+      }
+    }
+    C q[NF][QN][KB][VZ];                       // level-1 planes; plane p in slot (p - s_first) mod QN
+    C pb[PD][NB > 0 ? NB : 1][NVB][VZ];        // level-0 halo rings loaded PD steps ahead
+    auto load_ring = [&](int x, bool live, C (&dst)[NB > 0 ? NB : 1][NVB][VZ]) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        bool ok;
+        const Rsrc r = in_plane(P::BF[b], x, ok);
+        ok = ok && live;
+#pragma unroll
+        for (int j = 0; j < NVB; ++j) {
+          const Vec<S, VZ> w = bload<S, VZ>(r, ok ? roff[b][j] : BUF_OOB);
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) dst[b][j][v] = Cvt<S, C>::ld(w.v[v]);
+        }
+      }
+    };
+    // rings loaded in the step loop (planes s_first + 1 + PD, ...) and the
+    // planes stored (s_first - D, ...; from xb on)
+    PlaneWalk wr, wo;
+    wr.init(s_first + 1 + PD, a, w0, plane);
+    wo.init(s_first - D, a, false, plane);
+    const i64 xb_off = (i64)(a.h0 + xb) * plane;
+    auto load_ringw = [&](bool live, C (&dst)[NB > 0 ? NB : 1][NVB][VZ]) {
+      const bool ok = live && wr.ok(a);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const Rsrc r = make_rsrc(reinterpret_cast<const S*>(a.in[P::BF[b]]) + (ok ? wr.off : xb_off), pbytes);
+#pragma unroll
+        for (int j = 0; j < NVB; ++j) {
+          const Vec<S, VZ> w = bload<S, VZ>(r, ok ? roff[b][j] : BUF_OOB);
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) dst[b][j][v] = Cvt<S, C>::ld(w.v[v]);
+        }
+      }
+    };
+    auto put_ring = [&](C* dst, const C (&src)[NB > 0 ? NB : 1][NVB][VZ]) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int j = 0; j < NVB; ++j)
+          if (loff[b][j] >= 0) {
+#pragma unroll
+            for (int v = 0; v < VZ; ++v) dst[loff[b][j] + v] = src[b][j][v];
+          }
+    };
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int i = 0; i < QN; ++i)
+#pragma unroll
+        for (int k = 0; k < KB; ++k)
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) q[f][i][k][v] = C(0);
+#pragma unroll
+    for (int p = 0; p < PD; ++p) load_ring(s_first + p, true, pb[p]);
+    put_ring(lds, pb[0]);                      // plane s_first
+    load_ring(s_first + PD, s_first + PD <= s_last, pb[0]);
+    __syncthreads();
+
+    auto step = [&](auto Rc, int s) {
+      constexpr int r = decltype(Rc)::value;
+      constexpr int R = r % QN, PI = (r + 1) % PD;
+      // queue head: level-1 plane s - 1 (ring slot (r - 1) mod NR)
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int k = 0; k < KB; ++k)
+#pragma unroll
+          for (int v = 0; v < VZ; ++v)
+            q[f][pmod(R - 1, QN)][k][v] = nb[k][P::RY * RP + pmod(r - 1, NR) * SS + f * PR + P::RZ + v];
+      // level-0 halo ring of plane s + 1 into the other buffer, and the
+      // ring of plane s + 1 + PD loaded in its place
+      put_ring(lds + ((r + 1) & 1) * LE, pb[PI]);
+      load_ringw(s + 1 + PD <= s_last, pb[PI]);
+      wr.next(a, w0, plane, span);
+      // level 2 at plane p = s - D (stored from the chunk's first plane on)
+      const int p = s - D;
+      const bool st = p >= xb;
+      C res[NF][KB][VZ];
+#pragma unroll
+      for (int k = 0; k < KB; ++k)
+#pragma unroll
+        for (int v = 0; v < VZ; ++v) {
+          RingCell<P, R, KB, pmod(r - D, NR)> c{q, nb[k], k, v};
+          C o[NF];
+          P::eval(c, o);
+#pragma unroll
+          for (int m = 0; m < NF; ++m) res[m][k][v] = o[m];
+        }
+      const i64 po = st ? wo.off : xb_off;
+      wo.next(a, false, plane, span);
+#pragma unroll
+      for (int m = 0; m < NF; ++m) {
+        const Rsrc ro = make_rsrc(reinterpret_cast<S*>(a.out[m]) + po, pbytes);
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+          Vec<S, VZ> w;
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) w.v[v] = Cvt<S, C>::st(res[m][k][v]);
+          bstore<S, VZ, P::NTS ? BUF_NT : 0>(w, ro, st ? soff[k] : BUF_OOB);
+        }
+      }
+      __syncthreads();
+    };
+    for (int s = s_first; s <= s_last; s += UN) {
+      for_each_ic([&](auto Rc) {
+        if (s + decltype(Rc)::value <= s_last) step(Rc, s + decltype(Rc)::value);
+      }, typename mkseq<UN>::type{});
+    }
+  }
+}
+
 template <class P>
 ST_HD void tblock_entry(const SweepArgs& a) {
   const int n = a.ntz * a.nty * a.nch;
   const int t = xcd_swizzle((int)blockIdx.x, n, a.nxcd);
   const int tile_z = t % a.ntz;
   const int rest = t / a.ntz;
-  tblock_tile<P>(a, tile_z, rest % a.nty, rest / a.nty);
+  if constexpr (P::NA > 0) {
+    const int xb = (rest / a.nty) * a.xch;
+    tbws_tile<P>(a, tile_z, rest % a.nty, xb, xb + a.xch < a.n0 ? xb + a.xch : a.n0);
+  } else {
+    tblock_tile<P>(a, tile_z, rest % a.nty, rest / a.nty);
+  }
 }
 
 // ---------------------------------------------------------------------------

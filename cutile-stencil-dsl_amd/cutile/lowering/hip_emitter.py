@@ -128,6 +128,10 @@ class TBlockConfig:
     gy: int
     gz: int
     T: int
+    # > 0: the two-level kernel with wave roles (``st::tbws_tile``): ``na``
+    # threads evaluate level 1 on the region, the other ``nt - na`` level 2
+    # on the output tile
+    na: int = 0
 
     @property
     def oty(self) -> int:
@@ -139,10 +143,18 @@ class TBlockConfig:
 
     @property
     def kc(self) -> int:
-        return -(-(self.ty * (self.tz // self.vz)) // self.nt)
+        """Region vector cells per (level-1) thread."""
+        return -(-(self.ty * (self.tz // self.vz)) // (self.na or self.nt))
+
+    @property
+    def kb(self) -> int:
+        """Output vector cells per level-2 thread of the two-level kernel."""
+        return -(-(self.oty * (self.otz // self.vz)) // max(1, self.nt - self.na)) if self.na else 0
 
     @property
     def tag(self) -> str:
+        if self.na:
+            return f"tw{self.T}r{self.ty}x{self.tz}v{self.vz}n{self.nt}a{self.na}"
         return f"tb{self.T}r{self.ty}x{self.tz}v{self.vz}n{self.nt}"
 
     def kernel_config(self) -> KernelConfig:
@@ -245,6 +257,138 @@ def tblock_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
     return best[1]
 
 
+def tbws_eligible(meta: StencilMeta, T: int) -> bool:
+    """The two-level kernel with wave roles applies: two levels of a 3-D
+    closed system (result m is the next state of field m) whose in-plane
+    accesses all lie in the evaluated plane and whose column accesses reach
+    both neighbouring planes."""
+    if T != 2 or meta.ndim != 3 or meta.num_exprs != meta.num_inputs or meta.update:
+        return False
+    plan = plan_accesses(meta)
+    qlo, qhi = tblock_queue_range(meta)
+    return qlo <= -1 and qhi >= 1 and all(b[1] == 0 for b in plan["bufs"])
+
+
+def ring_pitch(cfg: TBlockConfig, elem_bytes: int, nr: int, nf: int) -> int:
+    """Elements per field and plane in a row of the two-level kernel's
+    level-1 ring (``nr`` planes of ``nf`` fields per row), as
+    ``st::ring_pitch``: the row span is congruent to the output width modulo
+    256 bytes (the level-2 threads' 16-byte reads free of bank conflicts),
+    unless an output row fills whole half-waves."""
+    if (cfg.otz // cfg.vz) % 32 == 0:
+        return cfg.tz
+    p = cfg.tz
+    while ((nr * nf * p - cfg.otz) * elem_bytes) % 256:
+        p += 1
+    return p
+
+
+def tbws_estimate(meta: StencilMeta, cfg: TBlockConfig, storage: str, compute: str) -> dict:
+    """LDS bytes (two level-0 planes and the level-1 ring) and a VGPR
+    estimate of the busier role of a two-level variant."""
+    plan = plan_accesses(meta)
+    qlo, qhi = tblock_queue_range(meta)
+    qn = qhi - qlo + 1
+    nr = qhi + 2
+    cb = ELEM_BYTES[compute]
+    cw = cb // 4
+    nf = max(1, meta.num_inputs)
+    lds = 2 * lds_bytes(meta, cfg.kernel_config(), compute) + nr * nf * cfg.ty * ring_pitch(cfg, cb, nr, nf) * cb
+    nbt = cfg.nt - cfg.na
+    nvb = 1
+    for f, dx, ry, rz in plan["bufs"]:
+        rzv = -(-rz // cfg.vz) * cfg.vz
+        w = (cfg.tz + 2 * rzv) // cfg.vz
+        ring = 2 * ry * w + cfg.ty * 2 * (rzv // cfg.vz)
+        nvb = max(nvb, -(-ring // nbt))
+    nb = max(1, len(plan["bufs"]))
+    # queues (+ two prefetched heads, or two prefetched rings), offsets, and
+    # the expression's loaded operands (about six vectors in flight; gfx950
+    # code objects of the 25-point star: 102-104 VGPRs at one cell each)
+    work = 6 * cw * cfg.vz + 24
+    reg_a = cw * cfg.vz * cfg.kc * nf * (qn + 2) + 8 * cfg.kc + work
+    reg_b = cw * cfg.vz * cfg.kb * nf * qn + 2 * cw * cfg.vz * nb * nvb + 4 * cfg.kb + 2 * nb * nvb + work
+    return {"lds": lds, "vgpr": max(reg_a, reg_b), "qn": qn}
+
+
+def tbws_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int,
+                budget: int = 160 * 1024, force: "tuple | None" = None) -> "TBlockConfig | None":
+    """Geometry of the two-level kernel with wave roles for vector width
+    ``vz``: fewest lanes per output cell (both roles, idle lanes included)
+    whose roles fit one workgroup (at most 1024 threads, whole waves, one
+    vector cell per thread) with the queues in the VGPRs its waves may use
+    and the planes in the LDS.  ``force`` = (output rows,
+    output columns, level-1 threads, level-2 threads) restricts the search
+    to that shape.  None when the kernel does not apply or nothing fits."""
+    if not tbws_eligible(meta, T):
+        return None
+    rx, ry, rz = radius3(meta)
+    gy = ry
+    gz = -(-rz // vz) * vz
+    best = None
+    otys, otzs = range(4, 65, 4), [t for t in (16, 24, 32, 40, 48, 64, 96, 128) if t % vz == 0]
+    if force is not None:
+        otys, otzs = (int(force[0]),), (int(force[1]),)
+        if otzs[0] % vz:
+            return None
+    for otz in otzs:
+        tz = otz + 2 * gz
+        for oty in otys:
+            ty = oty + 2 * gy
+            cells, ocells = ty * (tz // vz), oty * (otz // vz)
+            # two cells per thread only when forced: on gfx950 their doubled
+            # queues spill at the workgroup sizes that give the region one
+            # thread per cell
+            for ka in ((1, 2) if force is not None else (1,)):
+                na = -(-cells // ka)
+                na = -(-na // 64) * 64
+                for kb in ((1, 2) if force is not None else (1,)):
+                    nbt = -(-ocells // kb)
+                    nbt = -(-nbt // 64) * 64
+                    if force is not None:
+                        na, nbt = int(force[2]), int(force[3])
+                        if na % 64 or nbt % 64 or na <= 0 or nbt <= 0:
+                            return None
+                    nt = na + nbt
+                    if nt > 1024:
+                        continue
+                    cfg = TBlockConfig(ty, tz, vz, nt, gy, gz, T, na)
+                    if cfg.kc > 2 or cfg.kb > 2:
+                        continue
+                    est = tbws_estimate(meta, cfg, storage, compute)
+                    if est["lds"] > budget or (force is None and est["vgpr"] > 0.9 * _vgpr_limit(nt)):
+                        continue
+                    # lanes of both roles per output cell (gfx950, 25-point star
+                    # at 512^3 per depth-2 launch: FP64 24x32 tiles 771 us,
+                    # 16x40 840, 16x32 856; FP32 24x64 391 us, 16x96 424)
+                    cost = (na * cfg.kc + nbt * cfg.kb) * vz / (oty * otz)
+                    # ties: the larger tile, then the wider one (longer
+                    # contiguous rows, fewer halo columns per row)
+                    key = (round(cost, 3), -(oty * otz), -otz, nt)
+                    if best is None or key < best[0]:
+                        best = (key, cfg)
+    return None if best is None else best[1]
+
+
+def select_tblock(meta: StencilMeta, T: int, storage: str, compute: str, vz: int,
+                  hints: "dict | None" = None) -> "TBlockConfig | None":
+    """The register/LDS temporal-blocking variant the emitter builds for
+    depth ``T`` at vector width ``vz``: for two levels of a 3-D stencil
+    wider than one cell, the kernel with wave roles (hint ``tbws`` 0/1
+    overrides the choice, ``tbws_tile`` forces its geometry); otherwise, or
+    when that kernel does not apply, the single-role kernel (``tblock_tile``
+    forces its geometry).  None: neither fits."""
+    hints = hints or {}
+    ws = hints.get("tbws")
+    ws = max(radius3(meta)[1:]) >= 2 if ws is None else bool(ws)
+    tb = None
+    if ws and tbws_eligible(meta, T):
+        tb = tbws_config(meta, T, storage, compute, vz, force=hints.get("tbws_tile"))
+    if tb is None:
+        tb = tblock_config(meta, T, storage, compute, vz, force=hints.get("tblock_tile"))
+    return tb
+
+
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,
                 nontemporal: bool = True, prefetch: int = 1, partial: bool = False,
                 ntload: bool = False, tblock: "TBlockConfig | None" = None, brick: tuple = (0, 0, 0),
@@ -281,7 +425,7 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
     ]
     if tblock is not None:
         lines.append(f"  static constexpr int TT = {tblock.T}, GY = {tblock.gy}, GZ = {tblock.gz}, "
-                     f"TBSKIP = {int(tbskip)}, TBDB = {int(tbdb)}, TBPD = {int(tbpd)};")
+                     f"TBSKIP = {int(tbskip)}, TBDB = {int(tbdb)}, TBPD = {int(tbpd)}, NA = {int(tblock.na)};")
     upd = meta.update
     ne = meta.num_exprs
     if upd is None:
@@ -371,7 +515,7 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
         # the scalar variants (ragged rows): their queues already hold up to
         # 250 registers, and the second set's addresses (or a second
         # prefetch stage) spill them
-        fits = 2 * tblock_estimate(meta, tb, storage, compute)["lds"] <= 160 * 1024
+        fits = tb.na == 0 and 2 * tblock_estimate(meta, tb, storage, compute)["lds"] <= 160 * 1024
         db = int(fits and (bool(tbdb) if tbdb is not None else tb.vz > 1))
         # the deeper prefetch likewise only for the vector variants
         out.append(emit_traits(meta, tb.kernel_config(), storage, compute, tag, nontemporal, tblock=tb,

@@ -27,8 +27,8 @@ from typing import Optional, Sequence
 from cutile.frontend.ir import find_func
 from cutile.lowering.emitter import CodeEmitter
 from cutile.lowering.hip_emitter import (ELEM_BYTES, KernelConfig, brick_symbol, emit_hip_source, kernel_symbol,
-                                        lds_bytes, plan_accesses, radius3, sweep_tag, tblock_config, tblock_queue_range,
-                                        tblock_symbol, temporal_symbol, temporal_tile)
+                                        lds_bytes, plan_accesses, radius3, sweep_tag, tblock_queue_range,
+                                        select_tblock, tblock_symbol, temporal_symbol, temporal_tile)
 from cutile.lowering.meta import StencilMeta, apply_update, extract_fused_meta, extract_meta
 
 
@@ -133,10 +133,12 @@ def emit_module(meta: StencilMeta, tile_sizes: Sequence[int], halo_widths: Seque
         # register/LDS kernel at the vector width and at scalar width (arrays
         # whose rows are not vector-aligned); the LDS-ring kernel when the
         # ghost ring of T levels does not fit the registers or the LDS
-        force = (kernel_hints or {}).get("tblock_tile")
+        # (a forced geometry applies to the vector variant only)
+        hints = dict(kernel_hints or {})
         for vz in dict.fromkeys((min(vec.vz, 4), 1)):
-            tb = tblock_config(meta, int(temporal_steps), storage_t, compute_t, vz,
-                               force=force if vz > 1 or vec.vz == 1 else None)
+            h = hints if vz > 1 or vec.vz == 1 else {k: v for k, v in hints.items()
+                                                      if k not in ("tblock_tile", "tbws_tile")}
+            tb = select_tblock(meta, int(temporal_steps), storage_t, compute_t, vz, h)
             if tb is None:
                 tblocks = []
                 break

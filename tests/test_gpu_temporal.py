@@ -28,26 +28,45 @@ def _chain_zero_halo(oracle, u, fn, ndim, halo, T):
     return cur
 
 
-@pytest.mark.parametrize("which,T,skip", [("heat2d", 4, 0), ("diffusion7", 2, 0), ("diffusion7", 3, 0), ("star25", 2, 0),
-                                          # the per-wave level skip (option tbskip 1 and 2)
-                                          ("heat2d", 4, 1), ("diffusion7", 3, 1), ("star25", 2, 1),
-                                          ("heat2d", 4, 2), ("diffusion7", 3, 2)])
-def test_zero_mode_equals_zero_halo_chain(oracle, which, T, skip):
+ZERO_CASES = [
+    ("heat2d", 4, None), ("diffusion7", 2, None), ("diffusion7", 3, None), ("star25", 2, None),
+    # the per-wave level skip of the single-role kernel (option tbskip 1 and 2)
+    ("heat2d", 4, {"tbskip": 1}), ("diffusion7", 3, {"tbskip": 1}), ("star25", 2, {"tbskip": 1, "tbws": 0}),
+    ("heat2d", 4, {"tbskip": 2}), ("diffusion7", 3, {"tbskip": 2}),
+    # the two-level kernel with wave roles (the 25-point star's default at
+    # depth 2) against the single-role kernel, forced geometries, and the
+    # 7-point star forced onto it
+    ("star25", 2, {"tbws": 0}), ("star25", 2, {"tbws_tile": [24, 32, 640, 384]}),
+    ("star25", 2, {"tbws_tile": [24, 32, 320, 192]}), ("diffusion7", 2, {"tbws": 1}),
+    ("star25-float32", 2, None), ("star25-float32", 2, {"tbws_tile": [24, 64, 576, 384]}),
+]
+
+
+@pytest.mark.parametrize("which,T,hints", ZERO_CASES,
+                         ids=[f"{c[0]}-T{c[1]}-{'-'.join(f'{k}{v}' for k, v in (c[2] or {}).items()) or 'default'}"
+                              .replace(" ", "").replace("[", "").replace("]", "").replace(",", "x")
+                              for c in ZERO_CASES])
+def test_zero_mode_equals_zero_halo_chain(oracle, which, T, hints):
     import torch
 
     from cutile import compile
     from cutile import workloads as w
 
-    fn = getattr(w, which)()
-    r = compile(fn, temporal_steps=T, kernel_hints={"tbskip": skip} if skip else None)
+    name, _, dt = which.partition("-")
+    fn = getattr(w, name)(dt or "float64")
+    r = compile(fn, temporal_steps=T, kernel_hints=hints)
     v = r.load_module()._VARIANTS
     key = "tblock" if "tblock" in v else "temporal"
     assert r.temporal_steps == T and key in v
-    print(f"{which} T={T} skip {skip} variant {v[key]}", flush=True)
+    if hints and ("tbws_tile" in hints or hints.get("tbws")):
+        assert "_tblock_tw" in v[key][0], v[key]
+    if hints and hints.get("tbws") == 0:
+        assert "_tblock_tb" in v[key][0], v[key]
+    print(f"{which} T={T} hints {hints} variant {v[key]}", flush=True)
     rng = np.random.default_rng(T)
     base = {2: (70, 150), 3: (23, 19, 45)}[r.ndim]
     shape = tuple(b + 2 * h for b, h in zip(base, r.halo_widths))
-    u = rng.random(shape)
+    u = rng.random(shape).astype(dt or "float64")      # the oracle computes in the array's precision
     du = to_dev(u)
     out = torch.zeros_like(du)
     r.launcher(du, out)
@@ -57,18 +76,23 @@ def test_zero_mode_equals_zero_halo_chain(oracle, which, T, skip):
     assert np.array_equal(got[inner], ref[inner])
 
 
-@pytest.mark.parametrize("which,T,steps,shape", [
-    ("star25", 2, 5, (24, 20, 36)),
-    ("star25", 4, 6, (20, 24, 40)),
-    ("diffusion7", 4, 9, (30, 18, 50)),
+@pytest.mark.parametrize("which,T,steps,shape,hints", [
+    ("star25", 2, 5, (24, 20, 36), None),
+    ("star25", 4, 6, (20, 24, 40), None),
+    ("diffusion7", 4, 9, (30, 18, 50), None),
+    # blocks of the two-level kernel at a ragged shape (partial output tiles,
+    # rows read around every periodic axis), forced geometries
+    ("star25", 2, 7, (37, 45, 58), {"tbws_tile": [24, 32, 320, 192]}),
+    ("star25", 2, 7, (37, 45, 58), {"tbws_tile": [16, 32, 512, 256]}),
+    ("diffusion7", 2, 7, (21, 45, 58), {"tbws": 1}),
 ])
-def test_periodic_recompute_march(oracle, which, T, steps, shape):
+def test_periodic_recompute_march(oracle, which, T, steps, shape, hints):
     from cutile import workloads as w
     from cutile.runtime.timestep import TimeStepper
 
     fn = getattr(w, which)()
     spec = w.periodic_spec(3)
-    ts = TimeStepper(fn, spec, temporal_steps=T, split="fit")
+    ts = TimeStepper(fn, spec, temporal_steps=T, split="fit", block_hints=hints)
     assert ts.T == T and ts.mode == "recompute"
     halo = ts.halo
     print(f"{which} T={T} halo {halo} steps {steps}", flush=True)

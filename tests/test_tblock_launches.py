@@ -16,7 +16,7 @@ import tblock_model as tm
 from cutile import compile
 from cutile import workloads as w
 from cutile.frontend.ir import find_func
-from cutile.lowering.hip_emitter import (plan_accesses, tblock_config, tblock_queue_range, tblock_symbol)
+from cutile.lowering.hip_emitter import (plan_accesses, select_tblock, tblock_queue_range, tblock_symbol)
 from cutile.lowering.meta import extract_meta
 from cutile.runtime import kernel as K
 
@@ -49,10 +49,12 @@ def _launch_geometry(r, shape, storage):
     return kern, key, args
 
 
-def _cfg_for(fn, r, key, storage):
+def _cfg_for(fn, r, key, storage, hints=None):
     meta = extract_meta(find_func(fn._ir))
     vz = r.load_module()._KERNEL.variants[key][3]
-    cfg = tblock_config(meta, r.temporal_steps, storage, storage if storage != "bfloat16" else "float32", vz)
+    if hints and vz == 1 and r.load_module()._KERNEL.variants["vec"][3] > 1:
+        hints = {k: v for k, v in hints.items() if k not in ("tblock_tile", "tbws_tile")}
+    cfg = select_tblock(meta, r.temporal_steps, storage, storage if storage != "bfloat16" else "float32", vz, hints)
     assert tblock_symbol(meta, cfg) == r.load_module()._KERNEL.variants[key][0]
     return meta, cfg
 
@@ -75,15 +77,27 @@ DEVICE_CASES = [
     ("star25", 2, (20, 24, 40), "wrap"),          # a depth-4 block split into two depth-2 launches
     ("diffusion7", 4, (30, 18, 50), "wrap"),
     ("diffusion7", 2, (16, 20, 24), "wrap"),
+    # the two-level kernel: hints of tests/test_gpu_temporal.py
+    ("star25", 2, (23, 19, 45), 0, {"tbws": 0}),
+    ("star25", 2, (23, 19, 45), 0, {"tbws_tile": [24, 32, 640, 384]}),
+    ("star25", 2, (23, 19, 45), 0, {"tbws_tile": [24, 32, 320, 192]}),
+    ("diffusion7", 2, (23, 19, 45), 0, {"tbws": 1}),
+    ("star25-float32", 2, (23, 19, 45), 0, None),
+    ("star25-float32", 2, (23, 19, 45), 0, {"tbws_tile": [24, 64, 576, 384]}),
+    ("star25", 2, (37, 45, 58), "wrap", {"tbws_tile": [24, 32, 320, 192]}),
+    ("star25", 2, (37, 45, 58), "wrap", {"tbws_tile": [16, 32, 512, 256]}),
+    ("diffusion7", 2, (21, 45, 58), "wrap", {"tbws": 1}),
 ]
+DEVICE_CASES = [c if len(c) == 5 else c + (None,) for c in DEVICE_CASES]
 
 
-@pytest.mark.parametrize("which,T,n,tmode", DEVICE_CASES,
+@pytest.mark.parametrize("which,T,n,tmode,hints", DEVICE_CASES,
                          ids=[f"{c[0]}-T{c[1]}-{c[2] if isinstance(c[2], str) else 'x'.join(map(str, c[2]))}-m{c[3]}"
-                              for c in DEVICE_CASES])
-def test_device_case_matches_oracle(which, T, n, tmode):
-    fn = getattr(w, which)()
-    r = compile(fn, temporal_steps=T, temporal_mode="zero" if tmode == 0 else "recompute")
+                              f"{'-' + str(sorted(c[4].items())) if c[4] else ''}" for c in DEVICE_CASES])
+def test_device_case_matches_oracle(which, T, n, tmode, hints):
+    name, _, dt = which.partition("-")
+    fn = getattr(w, name)(dt or "float64")
+    r = compile(fn, temporal_steps=T, temporal_mode="zero" if tmode == 0 else "recompute", kernel_hints=hints)
     wrap = 0
     if tmode == "wrap":
         tmode, wrap = 0, {1: 4, 2: 5, 3: 7}[r.ndim]
@@ -92,8 +106,8 @@ def test_device_case_matches_oracle(which, T, n, tmode):
         n = tuple(64 + 2 * T * h - 2 * h for h in halo)
     shape = tuple(a + 2 * h for a, h in zip(n, halo))
     kern, key, args = _launch_geometry(r, shape, r.storage)
-    meta, cfg = _cfg_for(fn, r, key, r.storage)
-    u = np.random.default_rng(T).random(shape)
+    meta, cfg = _cfg_for(fn, r, key, r.storage, hints)
+    u = np.random.default_rng(T).random(shape)     # the model computes in float64 (any storage's geometry)
     ndim = len(n)
     if tmode == 0 and not wrap:
         from test_tblock_model import _ref_zero
@@ -195,7 +209,7 @@ def test_fused_block_launch_in_bounds(dtype, T, n, wrap):
     args, _ = kern.args(ins, ins, halo, key)
     meta = extract_fused_meta([f._ir for f in fns])
     vz = kern.variants[key][3]
-    cfg = tblock_config(meta, T, blk.storage, blk.storage, vz)
+    cfg = select_tblock(meta, T, blk.storage, blk.storage, vz)
     assert tblock_symbol(meta, cfg) == kern.variants[key][0]
     names = list(meta.input_names)
     argmap = [tm._arg_fields(f._fn, 2, names) for f in fns]

@@ -15,7 +15,7 @@ import tblock_model as tm
 from cutile import workloads as w
 from cutile.frontend.decorator import stencil_from_source
 from cutile.frontend.ir import find_func
-from cutile.lowering.hip_emitter import plan_accesses, radius3, tblock_config, tblock_queue_range
+from cutile.lowering.hip_emitter import plan_accesses, radius3, select_tblock, tblock_config, tblock_queue_range
 from cutile.lowering.meta import extract_meta
 
 
@@ -35,9 +35,10 @@ def line1d(u, i):
 """)
 
 
-def _setup(fn, T, storage, vz):
+def _setup(fn, T, storage, vz, hints=None):
+    """The variant the emitter builds (``hints``: its kernel hints)."""
     meta = extract_meta(find_func(fn._ir))
-    cfg = tblock_config(meta, T, storage, storage, vz)
+    cfg = select_tblock(meta, T, storage, storage, vz, hints)
     assert cfg is not None
     return meta, cfg, plan_accesses(meta), tblock_queue_range(meta)
 
@@ -66,18 +67,30 @@ CASES = [
     ("star25-T2-v2", w.star25, 2, "float64", 2, (10, 40, 36), (8, 8, 16)),
     ("star25-T2-v1", w.star25, 2, "float64", 1, (10, 37, 35), (8, 8, 8)),
     ("star25f-T2-v4", w.star25, 2, "float32", 4, (10, 56, 72), (8, 8, 32)),
+    # the two-level kernel with wave roles: the default for the 25-point
+    # star at depth 2 (above), forced for the 7-point star and forced
+    # geometries with two cells per thread
+    ("star25-T2-v2-ws24x32", w.star25, 2, "float64", 2, (9, 50, 70), (8, 8, 16), {"tbws_tile": (24, 32, 640, 384)}),
+    ("star25-T2-v2-ws2c", w.star25, 2, "float64", 2, (9, 50, 70), (8, 8, 16), {"tbws_tile": (24, 32, 320, 192)}),
+    ("diff7-T2-v2-ws", w.diffusion7, 2, "float64", 2, (9, 30, 200), (2, 2, 16), {"tbws": 1}),
+    ("diff7-T2-v1-ws", w.diffusion7, 2, "float64", 1, (9, 29, 101), (2, 2, 2), {"tbws": 1}),
+    ("diff7-T2-v2-tb", w.diffusion7, 2, "float64", 2, (9, 50, 70), (2, 2, 16), {"tbws": 0}),
+    ("star25-T2-v2-tb", w.star25, 2, "float64", 2, (10, 40, 36), (8, 8, 16), {"tbws": 0}),
     ("box27-T2-v2", box27, 2, "float64", 2, (8, 20, 30), (2, 2, 16)),
     ("box27-T3-v1", box27, 3, "float64", 1, (8, 21, 31), (3, 3, 3)),
     ("line1d-T3-v1", line1d, 3, "float64", 1, (50,), (6,)),
 ]
 
 
+CASES = [c if len(c) == 8 else c + (None,) for c in CASES]
+
+
 @pytest.mark.parametrize("mode", ["zero", "recompute", "wrap"])
-@pytest.mark.parametrize("name,make,T,storage,vz,n,halo", CASES, ids=[c[0] for c in CASES])
-def test_model_matches_oracle(name, make, T, storage, vz, n, halo, mode):
+@pytest.mark.parametrize("name,make,T,storage,vz,n,halo,hints", CASES, ids=[c[0] for c in CASES])
+def test_model_matches_oracle(name, make, T, storage, vz, n, halo, hints, mode):
     fn = make()
     ndim = len(n)
-    meta, cfg, plan, qr = _setup(fn, T, storage, vz)
+    meta, cfg, plan, qr = _setup(fn, T, storage, vz, hints)
     rng = np.random.default_rng(len(name) + T)
     shape = tuple(a + 2 * h for a, h in zip(n, halo))
     u = rng.random(shape)
@@ -100,16 +113,24 @@ def test_model_matches_oracle(name, make, T, storage, vz, n, halo, mode):
         np.testing.assert_array_equal(got[inner], ref[inner], err_msg=f"xch={xch}")
 
 
-@pytest.mark.parametrize("name,make,T,storage,vz,n,halo", CASES, ids=[c[0] for c in CASES])
-def test_geometry_invariants(name, make, T, storage, vz, n, halo):
+@pytest.mark.parametrize("name,make,T,storage,vz,n,halo,hints", CASES, ids=[c[0] for c in CASES])
+def test_geometry_invariants(name, make, T, storage, vz, n, halo, hints):
     fn = make()
-    meta, cfg, plan, (qlo, qhi) = _setup(fn, T, storage, vz)
+    meta, cfg, plan, (qlo, qhi) = _setup(fn, T, storage, vz, hints)
     rx, ry, rz = radius3(meta)
     assert qlo <= 0 <= qhi and qhi - qlo >= 2 * rx if rx else True
     assert cfg.gy >= (T - 1) * ry and cfg.gz >= (T - 1) * rz
     assert cfg.gz % cfg.vz == 0 and cfg.tz % cfg.vz == 0 and cfg.otz % cfg.vz == 0
     assert cfg.oty > 0 and cfg.otz > 0
-    assert cfg.kc * cfg.nt >= cfg.ty * cfg.tz // cfg.vz > (cfg.kc - 1) * cfg.nt
+    if cfg.na:
+        # roles of whole waves; every region / output vector has a thread
+        nb = cfg.nt - cfg.na
+        assert cfg.na % 64 == 0 and nb % 64 == 0 and cfg.nt <= 1024
+        assert cfg.kc * cfg.na >= cfg.ty * cfg.tz // cfg.vz > (cfg.kc - 1) * cfg.na
+        assert cfg.kb * nb >= cfg.oty * cfg.otz // cfg.vz > (cfg.kb - 1) * nb
+        assert T == 2 and cfg.gy >= ry and cfg.gz >= rz
+    else:
+        assert cfg.kc * cfg.nt >= cfg.ty * cfg.tz // cfg.vz > (cfg.kc - 1) * cfg.nt
 
 
 def _ref_zero_multi(fields, fns, names, ndim, halo, T):
