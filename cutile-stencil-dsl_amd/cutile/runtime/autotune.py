@@ -377,11 +377,14 @@ def autotune_split(stepper, u, min_ms: float = 20.0, verbose: bool = False) -> i
     """The depth d (a divisor of ``stepper.T``; 1 = single sweeps) whose
     launches advance arrays shaped like ``u`` fastest per step.  Each
     candidate runs whole blocks of T steps, captured in a hipGraph and
-    replayed as step loops run, on two scratch arrays of u's shape (zeros,
-    the stepper's boundary handling) for about ``min_ms``; depths whose
-    levels fit neither registers nor LDS are skipped.  The choice is kept in
-    the tuned tables per (stencil, storage, size bucket, T, mode, arch).
-    Every candidate gives the same states."""
+    replayed as step loops run, on two scratch copies of ``u`` (its values
+    matter: the arithmetic-bound temporal kernels draw less power on an
+    all-zero field and keep a higher clock -- on MI355X the 25-point star's
+    depth-2 kernel ran 16 % faster on zeros than on the benchmark's field,
+    the memory-bound sweep 1 %) for about ``min_ms``; depths whose levels
+    fit neither registers nor LDS are skipped.  The choice is kept in the
+    tuned tables per (stencil, storage, size bucket, T, mode, arch).  Every
+    candidate gives the same states."""
     from cutile.frontend.ir import find_func
 
     T = stepper.T
@@ -399,8 +402,8 @@ def _measure_split(stepper, u, min_ms: float, verbose: bool) -> int:
     from cutile.runtime import rt
 
     T = stepper.T
-    a = torch.zeros_like(u)
-    b = torch.zeros_like(u)
+    a = u.clone()
+    b = u.clone()
     best = None
     saved = dict(stepper._plans)
     side = torch.cuda.Stream()
