@@ -115,6 +115,10 @@ ST_HD int xcd_swizzle(int i, int n, int nxcd) {
   return x * q + (x < r ? x : r) + j;
 }
 
+constexpr int pmod(int a, int n) { return ((a % n) + n) % n; }
+constexpr int gcd_i(int a, int b) { return b == 0 ? a : gcd_i(b, a % b); }
+constexpr int lcm_i(int a, int b) { return a / gcd_i(a, b) * b; }
+
 // compile-time integer sequences (hiprtc has no <utility>)
 template <int... I> struct iseq {};
 template <int N, int... I> struct mkseq : mkseq<N - 1, N - 1, I...> {};
@@ -162,6 +166,9 @@ template <class P> struct Tab {
   static constexpr int nvb(int b) { return (gvecs(b) + P::NT - 1) / P::NT; }
   static constexpr int off(int b) { int o = 0; for (int k = 0; k < b; ++k) o += rows(k) * pitch(k); return o; }
   static constexpr int lds_elems() { int o = 0; for (int k = 0; k < P::NB; ++k) o += rows(k) * pitch(k); return o > 0 ? o : 1; }
+  // LDS of a sweep: one set of plane buffers, or two (P::SDB) used by
+  // alternate planes
+  static constexpr int sweep_elems() { return (P::SDB ? 2 : 1) * lds_elems(); }
   static constexpr int max_nvb() { int m = 1; for (int k = 0; k < P::NB; ++k) m = nvb(k) > m ? nvb(k) : m; return m; }
   static constexpr int buf(int f, int dx) {
     for (int k = 0; k < P::NB; ++k) if (P::BF[k] == f && P::BDX[k] == dx) return k;
@@ -188,7 +195,7 @@ template <class P> struct Tab {
 // the number of cells per thread (the queue's third extent).
 // bytes of LDS the largest of several sweep instantiations needs
 template <class... P> constexpr int sweep_lds_bytes() {
-  const int bytes[] = {16, (int)(Tab<P>::lds_elems() * sizeof(typename P::C))...};
+  const int bytes[] = {16, (int)(Tab<P>::sweep_elems() * sizeof(typename P::C))...};
   int m = 0;
   for (int b : bytes) m = b > m ? b : m;
   return m;
@@ -318,7 +325,7 @@ ST_HD int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 // along it wrap around instead of reading the halo, so no halo fill is
 // needed between steps (the output's halo is not written).  The host allows
 // it when n_d >= h_d and, for the vector variant, n2 % VZ == 0.
-// lds: the workgroup's LDS, at least Tab<P>::lds_elems() elements (kernels
+// lds: the workgroup's LDS, at least Tab<P>::sweep_elems() elements (kernels
 // that dispatch to several instantiations share one pool sized to the largest)
 template <class P>
 ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typename P::C* lds) {
@@ -346,9 +353,13 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
   const bool zload = a.h2 + zq + VZ <= s2;      // ... and it lies inside the array
 
   constexpr int PD = P::PD;   // planes prefetched ahead into registers
+  // halo rings loaded RPD planes ahead (1: the next plane).  Fetching a
+  // plane's ring when the neighbouring tiles fetch it as their queue head
+  // (RPD = QHI + PD) lets the shared rows hit in the XCD's L2.
+  constexpr int RPD = P::RPD < 1 ? 1 : P::RPD;
   C q[NF][QN][KY][VZ];
   C pq[PD][NF][KY][VZ];
-  C pb[NB > 0 ? NB : 1][NVB][VZ];
+  C pbq[RPD][NB > 0 ? NB : 1][NVB][VZ];
 
   auto in_rsrc = [&](int f, int x) -> Rsrc {
     const int px = w0 ? wrapi(x, a.n0) : x;
@@ -403,7 +414,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
   };
 
   // prefetch the memory part (halo ring, or whole tile) of every LDS buffer
-  auto load_bufs = [&](int x, bool live) {
+  auto load_bufs = [&](int x, bool live, C (&pb)[NB > 0 ? NB : 1][NVB][VZ]) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const Rsrc r = in_rsrc(P::BF[b], x + P::BDX[b]);
@@ -416,11 +427,11 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
     }
   };
 
-  auto store_bufs = [&](auto Rc) {
+  auto store_bufs = [&](auto Rc, const C (&pb)[NB > 0 ? NB : 1][NVB][VZ], C* ldsb) {
     constexpr int R = decltype(Rc)::value;
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      C* base = lds + T::off(b);
+      C* base = ldsb + T::off(b);
       if constexpr (T::HASQ) {
         if (T::fromq(b)) {
 #pragma unroll
@@ -459,12 +470,20 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
       for (int k = 0; k < PD; ++k) load_cells(f, xb + P::QHI + k, pq[k][f], xb + k < xe);
     }
   }
-  load_bufs(xb, true);
+#pragma unroll
+  for (int d = 0; d < RPD; ++d) load_bufs(xb + d, xb + d < xe, pbq[d]);
+
+  // P::SDB: two sets of LDS plane buffers used by alternate planes, so a
+  // plane's writes need no barrier after the previous plane's reads (one
+  // barrier per plane instead of two)
+  constexpr int NBUF = P::SDB ? 2 : 1;
+  constexpr int UN = lcm_i(QN, NBUF);
 
   // one plane: rotation R = (x - xb) % QN names the queue slots, so the
-  // queue never moves between registers
+  // queue never moves between registers (every use of R takes it modulo QN)
   auto step = [&](auto Rc, int x) {
     constexpr int R = decltype(Rc)::value;
+    C* const ldsb = lds + (NBUF == 2 ? (R & 1) : 0) * T::lds_elems();
     // install plane x+QHI in the slot that held plane x-1+QLO
     if constexpr (T::HASQ) {
 #pragma unroll
@@ -483,18 +502,26 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
       }
     }
     if constexpr (NB > 0) {
-      __syncthreads();
-      store_bufs(Rc);
+      if constexpr (NBUF == 1) __syncthreads();
+      store_bufs(Rc, pbq[0], ldsb);
       __syncthreads();
     }
-    // prefetch plane x+PD of the queue and the LDS ring of plane x+1 while
+    // prefetch plane x+PD of the queue and the LDS ring of plane x+RPD while
     // plane x is computed (a plane past the chunk is loaded as out of range)
     if constexpr (T::HASQ) {
 #pragma unroll
       for (int f = 0; f < NF; ++f)
         if (P::QF[f]) load_cells(f, x + PD + P::QHI, pq[PD - 1][f], x + PD < xe);
     }
-    load_bufs(x + 1, x + 1 < xe);
+#pragma unroll
+    for (int d = 0; d + 1 < RPD; ++d)
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int j = 0; j < NVB; ++j)
+#pragma unroll
+          for (int v = 0; v < VZ; ++v) pbq[d][b][j][v] = pbq[d + 1][b][j][v];
+    load_bufs(x + RPD, x + RPD < xe, pbq[RPD - 1]);
 
     Rsrc ro[NOUT];
 #pragma unroll
@@ -532,7 +559,7 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
       C res[NOUT][VZ];
 #pragma unroll
       for (int v = 0; v < VZ; ++v) {
-        Cell<P, R> c{q, lds, k, v, lyk, tz * VZ + v, zn, tz, tid & 63};
+        Cell<P, R> c{q, ldsb, k, v, lyk, tz * VZ + v, zn, tz, tid & 63};
         C o[NO];
         P::eval(c, o);
         if constexpr (P::UPD == 0) {
@@ -578,10 +605,10 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
     }
   };
 
-  for (int x = xb; x < xe; x += QN) {
+  for (int x = xb; x < xe; x += UN) {
     for_each_ic([&](auto Rc) {
       if (x + decltype(Rc)::value < xe) step(Rc, x + decltype(Rc)::value);
-    }, typename mkseq<QN>::type{});
+    }, typename mkseq<UN>::type{});
   }
 }
 
@@ -623,9 +650,6 @@ ST_HD void sweep_tile(const SweepArgs& a, int tile_z, int tile_y, int chunk, typ
 //   a.wrap bit d: axis d is periodic; level-0 reads past the interior wrap
 //            around, so intermediate levels there are exact periodic images
 //            (no halo is read or written along that axis).
-constexpr int pmod(int a, int n) { return ((a % n) + n) % n; }
-constexpr int gcd_i(int a, int b) { return b == 0 ? a : gcd_i(b, a % b); }
-constexpr int lcm_i(int a, int b) { return a / gcd_i(a, b) * b; }
 
 // index along a periodic axis of interior extent n (any distance)
 ST_HD int wrapm(int i, int n) {
@@ -1051,7 +1075,11 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
   constexpr int LE = T::lds_elems();               // one level-0 plane (every buffer)
   constexpr int PR = ring_pitch<P>(), SS = NF * PR, RP = NR * SS;   // ring: field, plane, row
   constexpr int PD = P::TBPD < 1 ? 1 : (P::TBPD > 4 ? 4 : P::TBPD);
-  constexpr int UN = lcm_i(QN, lcm_i(2, lcm_i(NR, PD)));
+  // level-0 halo rings are loaded RB planes ahead of their use (RB = QHI +
+  // PD: with the neighbouring tiles' queue heads of the same plane, so the
+  // shared rows hit in the XCD's L2)
+  constexpr int RB = P::TBRP < 1 ? PD : P::TBRP;
+  constexpr int UN = lcm_i(QN, lcm_i(2, lcm_i(NR, lcm_i(PD, RB))));
   static_assert(P::TT == 2 && P::NO == NF && P::UPD == 0, "two levels; result m is the next state of field m");
   static_assert(NA % 64 == 0 && NBT > 0 && NBT % 64 == 0, "roles are whole waves");
   static_assert(QLO <= -1 && QHI >= 1, "column accesses on both sides of the plane");
@@ -1254,7 +1282,7 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
       }
     }
     C q[NF][QN][KB][VZ];                       // level-1 planes; plane p in slot (p - s_first) mod QN
-    C pb[PD][NB > 0 ? NB : 1][NVB][VZ];        // level-0 halo rings loaded PD steps ahead
+    C pb[RB][NB > 0 ? NB : 1][NVB][VZ];        // level-0 halo rings loaded RB steps ahead
     auto load_ring = [&](int x, bool live, C (&dst)[NB > 0 ? NB : 1][NVB][VZ]) {
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
@@ -1269,10 +1297,10 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
         }
       }
     };
-    // rings loaded in the step loop (planes s_first + 1 + PD, ...) and the
+    // rings loaded in the step loop (planes s_first + 1 + RB, ...) and the
     // planes stored (s_first - D, ...; from xb on)
     PlaneWalk wr, wo;
-    wr.init(s_first + 1 + PD, a, w0, plane);
+    wr.init(s_first + 1 + RB, a, w0, plane);
     wo.init(s_first - D, a, false, plane);
     const i64 xb_off = (i64)(a.h0 + xb) * plane;
     auto load_ringw = [&](bool live, C (&dst)[NB > 0 ? NB : 1][NVB][VZ]) {
@@ -1307,14 +1335,14 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
 #pragma unroll
           for (int v = 0; v < VZ; ++v) q[f][i][k][v] = C(0);
 #pragma unroll
-    for (int p = 0; p < PD; ++p) load_ring(s_first + p, true, pb[p]);
+    for (int p = 0; p < RB; ++p) load_ring(s_first + p, s_first + p <= s_last, pb[p]);
     put_ring(lds, pb[0]);                      // plane s_first
-    load_ring(s_first + PD, s_first + PD <= s_last, pb[0]);
+    load_ring(s_first + RB, s_first + RB <= s_last, pb[0]);
     __syncthreads();
 
     auto step = [&](auto Rc, int s) {
       constexpr int r = decltype(Rc)::value;
-      constexpr int R = r % QN, PI = (r + 1) % PD;
+      constexpr int R = r % QN, PI = (r + 1) % RB;
       // queue head: level-1 plane s - 1 (ring slot (r - 1) mod NR)
 #pragma unroll
       for (int f = 0; f < NF; ++f)
@@ -1324,9 +1352,9 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
           for (int v = 0; v < VZ; ++v)
             q[f][pmod(R - 1, QN)][k][v] = nb[k][P::RY * RP + pmod(r - 1, NR) * SS + f * PR + P::RZ + v];
       // level-0 halo ring of plane s + 1 into the other buffer, and the
-      // ring of plane s + 1 + PD loaded in its place
+      // ring of plane s + 1 + RB loaded in its place
       put_ring(lds + ((r + 1) & 1) * LE, pb[PI]);
-      load_ringw(s + 1 + PD <= s_last, pb[PI]);
+      load_ringw(s + 1 + RB <= s_last, pb[PI]);
       wr.next(a, w0, plane, span);
       // level 2 at plane p = s - D (stored from the chunk's first plane on)
       const int p = s - D;
@@ -1501,7 +1529,7 @@ ST_HD void temporal_entry(const SweepArgs& a) {
 
 template <class P>
 ST_HD void sweep_entry(const SweepArgs& a) {
-  __shared__ typename P::C lds[Tab<P>::lds_elems()];
+  __shared__ typename P::C lds[Tab<P>::sweep_elems()];
   const int n = a.ntz * a.nty * a.nch;
   const int t = xcd_swizzle((int)blockIdx.x, n, a.nxcd);
   const int tile_z = t % a.ntz;

@@ -392,7 +392,8 @@ def select_tblock(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
 def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str, tag: str,
                 nontemporal: bool = True, prefetch: int = 1, partial: bool = False,
                 ntload: bool = False, tblock: "TBlockConfig | None" = None, brick: tuple = (0, 0, 0),
-                zdpp: bool = False, tbskip: int = 0, tbdb: int = 0, tbpd: int = 2) -> str:
+                zdpp: bool = False, tbskip: int = 0, tbdb: int = 0, tbpd: int = 2, ringpd: int = 1,
+                sdb: int = 0, tbrp: int = 0) -> str:
     """Traits struct of one kernel variant.  ``partial``: rows may end in a
     partial vector (batched instances whose interior is not a multiple of
     the vector width).  ``tblock``: traits of a temporal-blocking variant
@@ -419,13 +420,15 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
         f"  static constexpr int BRZ[{max(1, len(bufs))}] = {_arr([b[3] for b in bufs])};",
         f"  static constexpr int RX = {r3[0]}, RY = {r3[1]}, RZ = {r3[2]};",
         f"  static constexpr int NTS = {1 if nontemporal else 0}, PD = {int(prefetch)}, "
-        f"PARTIAL = {1 if partial else 0}, NTL = {1 if ntload else 0};",
+        f"PARTIAL = {1 if partial else 0}, NTL = {1 if ntload else 0}, RPD = {max(1, int(ringpd))}, "
+        f"SDB = {1 if sdb else 0};",
         f"  static constexpr int BR0 = {int(brick[0])}, BR1 = {int(brick[1])}, BR2 = {int(brick[2])};",
         f"  static constexpr int ZDPP = {1 if zdpp and cfg.vz > 0 else 0};",
     ]
     if tblock is not None:
         lines.append(f"  static constexpr int TT = {tblock.T}, GY = {tblock.gy}, GZ = {tblock.gz}, "
-                     f"TBSKIP = {int(tbskip)}, TBDB = {int(tbdb)}, TBPD = {int(tbpd)}, NA = {int(tblock.na)};")
+                     f"TBSKIP = {int(tbskip)}, TBDB = {int(tbdb)}, TBPD = {int(tbpd)}, NA = {int(tblock.na)}, "
+                     f"TBRP = {int(tbrp)};")
     upd = meta.update
     ne = meta.num_exprs
     if upd is None:
@@ -473,7 +476,8 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
                     compute: str, temporal=None, nontemporal: bool = True, prefetch: int = 1,
                     ntload: bool = False, tblocks: Sequence[TBlockConfig] = (),
                     bricked: Sequence[tuple] = (), zdpp: bool = False, tbskip: int = 0,
-                    tbdb: "int | None" = None, tbpd: int = 2) -> str:
+                    tbdb: "int | None" = None, tbpd: int = 2, ringpd: int = 1, sdb: int = 0,
+                    tbrp: "int | None" = None) -> str:
     """Full HIP translation unit with one sweep kernel per variant (plus the
     temporal-blocking kernels: ``temporal`` = (config, T) of the LDS-ring
     kernel, ``tblocks`` the register/LDS ones)."""
@@ -483,9 +487,10 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
         '#include "stencil_core.h"',
         "",
     ]
-    for cfg in variants:
+    for cfg in dict.fromkeys(variants):        # the vector and scalar variants may coincide
         tag = sweep_tag(cfg, zdpp)
-        out.append(emit_traits(meta, cfg, storage, compute, tag, nontemporal, prefetch, ntload=ntload, zdpp=zdpp))
+        out.append(emit_traits(meta, cfg, storage, compute, tag, nontemporal, prefetch, ntload=ntload, zdpp=zdpp,
+                               ringpd=ringpd, sdb=sdb))
         out.append(
             f'extern "C" __global__ void __launch_bounds__({cfg.nt}) '
             f"{kernel_symbol(meta, tag)}(st::SweepArgs a) {{ st::sweep_entry<P_{tag}>(a); }}"
@@ -518,8 +523,11 @@ def emit_hip_source(meta: StencilMeta, variants: Sequence[KernelConfig], storage
         fits = tb.na == 0 and 2 * tblock_estimate(meta, tb, storage, compute)["lds"] <= 160 * 1024
         db = int(fits and (bool(tbdb) if tbdb is not None else tb.vz > 1))
         # the deeper prefetch likewise only for the vector variants
+        pd = tbpd if tb.vz > 1 else 1
+        # the two-level kernel's halo rings: QHI + PD planes ahead (0: PD)
+        rp = (tblock_queue_range(meta)[1] + pd if tbrp is None else int(tbrp)) if tb.na else 0
         out.append(emit_traits(meta, tb.kernel_config(), storage, compute, tag, nontemporal, tblock=tb,
-                               zdpp=zdpp, tbskip=tbskip, tbdb=db, tbpd=tbpd if tb.vz > 1 else 1))
+                               zdpp=zdpp, tbskip=tbskip, tbdb=db, tbpd=pd, tbrp=rp))
         out.append(
             f'extern "C" __global__ void __launch_bounds__({tb.nt}) '
             f"{tblock_symbol(meta, tb)}(st::SweepArgs a) {{ st::tblock_entry<P_{tag}>(a); }}"

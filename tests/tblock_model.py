@@ -326,6 +326,7 @@ def run_fields_ws(fns, argmap, ndim, cfg, plan, qrange, us, halo, xch, tmode=0, 
     QN, D = QHI - QLO + 1, QHI + 1
     NR = D + 1
     PD = 2 if VZ > 1 else 1
+    RB = QHI + PD                 # the level-2 threads' halo rings: QHI + PD planes ahead
     errs = []
     if TT != 2 or not (QLO <= -1 and QHI >= 1):
         errs.append("two-level kernel: T = 2 and column accesses on both sides")
@@ -433,10 +434,10 @@ def run_fields_ws(fns, argmap, ndim, cfg, plan, qrange, us, halo, xch, tmode=0, 
             for i in range(QN - 1):
                 qa[fi][i] = load_q0(fi, s_first + QLO + i, True)
         pqa = [[load_q0(fi, s_first + QHI + p, True) for fi in range(NF)] for p in range(PD)]
-        pb = [load_ring(s_first + p, True) for p in range(PD)]
+        pb = [load_ring(s_first + p, s_first + p <= s_last) for p in range(RB)]
         put_l0(l0[0], -QLO)
         put_ring(l0[0], pb[0])
-        pb[0] = load_ring(s_first + PD, s_first + PD <= s_last)
+        pb[0] = load_ring(s_first + RB, s_first + RB <= s_last)
         for s in range(s_first, s_last + 1):
             r = s - s_first
             reads, writes = set(), set()
@@ -476,8 +477,8 @@ def run_fields_ws(fns, argmap, ndim, cfg, plan, qrange, us, halo, xch, tmode=0, 
             for fi in range(NF):
                 qb[fi][_pmod(r - 1, QN)] = ring[_pmod(r - 1, NR)][fi][osl].copy()
             reads.add(("ring", _pmod(r - 1, NR)))
-            put_ring(l0[(r + 1) & 1], pb[(r + 1) % PD])
-            pb[(r + 1) % PD] = load_ring(s + 1 + PD, s + 1 + PD <= s_last)
+            put_ring(l0[(r + 1) & 1], pb[(r + 1) % RB])
+            pb[(r + 1) % RB] = load_ring(s + 1 + RB, s + 1 + RB <= s_last)
             p = s - D
             sl = _pmod(r - D, NR)
 
