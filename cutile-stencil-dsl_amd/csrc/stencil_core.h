@@ -1082,6 +1082,7 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
   constexpr int UN = lcm_i(QN, lcm_i(2, lcm_i(NR, lcm_i(PD, RB))));
   static_assert(P::TT == 2 && P::NO == NF && P::UPD == 0, "two levels; result m is the next state of field m");
   static_assert(NA % 64 == 0 && NBT > 0 && NBT % 64 == 0, "roles are whole waves");
+  static_assert(!P::ZDPP, "cross-lane z-neighbours are an option of the sweep and the single-role kernel");
   static_assert(QLO <= -1 && QHI >= 1, "column accesses on both sides of the plane");
   static_assert(OTY > 0 && OTZ > 0 && GZ % VZ == 0 && TZ % VZ == 0 && GY >= P::RY && GZ >= P::RZ,
                 "two-level tile geometry");

@@ -375,12 +375,15 @@ def select_tblock(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
     """The register/LDS temporal-blocking variant the emitter builds for
     depth ``T`` at vector width ``vz``: for two levels of a 3-D stencil
     wider than one cell, the kernel with wave roles (hint ``tbws`` 0/1
-    overrides the choice, ``tbws_tile`` forces its geometry); otherwise, or
+    overrides the choice, ``tbws_tile`` forces its geometry; the ``zdpp``
+    hint, which that kernel does not take, selects the other); otherwise, or
     when that kernel does not apply, the single-role kernel (``tblock_tile``
     forces its geometry).  None: neither fits."""
     hints = hints or {}
     ws = hints.get("tbws")
     ws = max(radius3(meta)[1:]) >= 2 if ws is None else bool(ws)
+    if hints.get("zdpp"):
+        ws = False        # cross-lane z-neighbours: an option of the single-role kernel only
     tb = None
     if ws and tbws_eligible(meta, T):
         tb = tbws_config(meta, T, storage, compute, vz, force=hints.get("tbws_tile"))

@@ -189,3 +189,14 @@ def test_coupled_fields_model_matches_oracle(T, vz, n, halo, mode):
         assert not errs, (xch, errs[:5])
         for k, g in zip(names, got):
             np.testing.assert_array_equal(g[inner], ref[k][inner], err_msg=f"{k} xch={xch}")
+
+
+def test_zdpp_hint_selects_the_single_role_kernel():
+    """Cross-lane z-neighbours (hint ``zdpp``) are an option of the
+    single-role kernel: the two-level kernel does not take them (its
+    level-1 view has no neighbour-lane vectors), so the hint selects the
+    other kernel even where the two-level one is the default."""
+    meta = extract_meta(find_func(w.star25()._ir))
+    assert select_tblock(meta, 2, "float64", "float64", 2).na > 0
+    assert select_tblock(meta, 2, "float64", "float64", 2, {"zdpp": 1}).na == 0
+    assert select_tblock(meta, 2, "float64", "float64", 2, {"zdpp": 1, "tbws": 1}).na == 0
