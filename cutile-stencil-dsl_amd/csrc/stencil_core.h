@@ -218,7 +218,10 @@ template <int DIR, class C> ST_HD C lane_shift(C x) {
   }
 }
 
-template <class P, int R, int KD = Tab<P>::KY> struct Cell {
+// YA: the thread's KD cells lie on adjacent rows (cell k + d is d rows
+// below cell k), so an in-plane neighbour DY rows away that is another of
+// my cells is read from my registers, not from LDS.
+template <class P, int R, int KD = Tab<P>::KY, bool YA = false> struct Cell {
   using C = typename P::C;
   using T = Tab<P>;
   const C (*q)[T::QN][KD][P::VZ];
@@ -258,6 +261,13 @@ template <class P, int R, int KD = Tab<P>::KY> struct Cell {
     } else {
       constexpr int b = T::buf(F, DX);
       static_assert(b >= 0, "access has no LDS buffer");
+      if constexpr (YA && DY != 0 && DZ > -P::VZ && DZ < P::VZ && T::HASQ && DX >= P::QLO && DX <= P::QHI &&
+                    P::QF[F] && T::fromq(b)) {
+        // another of my cells (the LDS interior is copied from the queue);
+        // k and v are constants once the cell loops are unrolled
+        if (k + DY >= 0 && k + DY < KD && v + DZ >= 0 && v + DZ < P::VZ)
+          return q[F][(DX - P::QLO + R) % T::QN][k + DY][v + DZ];
+      }
       return lds[T::off(b) + (ly + P::BRY[b] + DY) * T::pitch(b) + (lz + T::rzv(b) + DZ)];
     }
   }
@@ -1036,7 +1046,7 @@ template <class P> constexpr int ring_pitch() {
 // `nb` points at the ring cell (row - RY, column - RZ) of my vector's first
 // cell in slot 0, so every in-plane access is a non-negative constant
 // offset from it (LDS instruction offsets are unsigned).
-template <class P, int R, int KD, int SL> struct RingCell {
+template <class P, int R, int KD, int SL, bool YA = false> struct RingCell {
   using C = typename P::C;
   using T = Tab<P>;
   static constexpr int PR = ring_pitch<P>();             // one field of one plane in a row
@@ -1055,6 +1065,10 @@ template <class P, int R, int KD, int SL> struct RingCell {
       if constexpr (DY == 0 && DZ > -P::VZ && DZ < P::VZ) {
         // a z-neighbour inside my own vector is in the queue
         if (v + DZ >= 0 && v + DZ < P::VZ) return q[F][(0 - P::QLO + R) % T::QN][k][v + DZ];
+      } else if constexpr (YA && DZ > -P::VZ && DZ < P::VZ) {
+        // a neighbour in another of my cells (adjacent rows): in the queue
+        if (k + DY >= 0 && k + DY < KD && v + DZ >= 0 && v + DZ < P::VZ)
+          return q[F][(0 - P::QLO + R) % T::QN][k + DY][v + DZ];
       }
       return nb[(DY + P::RY) * RP + SL * SS + F * PR + (v + DZ + P::RZ)];
     }
@@ -1083,6 +1097,13 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
   static_assert(P::TT == 2 && P::NO == NF && P::UPD == 0, "two levels; result m is the next state of field m");
   static_assert(NA % 64 == 0 && NBT > 0 && NBT % 64 == 0, "roles are whole waves");
   static_assert(!P::ZDPP, "cross-lane z-neighbours are an option of the sweep and the single-role kernel");
+  // P::TBYA: a thread's cells lie on adjacent rows of one column (level 1:
+  // KA rows of the region, level 2: KB rows of the output tile), so the
+  // in-plane neighbours they share are read from LDS once and the ones that
+  // are my own cells come from my queue
+  constexpr bool YA = P::TBYA != 0;
+  static_assert(!YA || (TY % KA == 0 && NA >= (TY / KA) * NTZ && OTY % KB == 0 && NBT >= (OTY / KB) * ONTZ),
+                "adjacent-row cells: whole row groups, one thread per group and column");
   static_assert(QLO <= -1 && QHI >= 1, "column accesses on both sides of the plane");
   static_assert(OTY > 0 && OTZ > 0 && GZ % VZ == 0 && TZ % VZ == 0 && GY >= P::RY && GZ >= P::RZ,
                 "two-level tile geometry");
@@ -1133,11 +1154,18 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
     bool act[KA], keep[KA];
 #pragma unroll
     for (int k = 0; k < KA; ++k) {
-      const int c = t + k * NA;
-      act[k] = c < CELLS;
-      const int cc = act[k] ? c : CELLS - 1;
-      ly[k] = cc / NTZ;
-      lz[k] = (cc % NTZ) * VZ;
+      if constexpr (YA) {
+        const int grp = t / NTZ;
+        act[k] = grp < TY / KA;
+        ly[k] = (act[k] ? grp : TY / KA - 1) * KA + k;
+        lz[k] = (t % NTZ) * VZ;
+      } else {
+        const int c = t + k * NA;
+        act[k] = c < CELLS;
+        const int cc = act[k] ? c : CELLS - 1;
+        ly[k] = cc / NTZ;
+        lz[k] = (cc % NTZ) * VZ;
+      }
       int y = y0 + ly[k], z = z0 + lz[k];
       keep[k] = (w1 || (y >= 0 && y < a.n1)) && (w2 || (z >= 0 && z < a.n2));
       if (w1) y = wrapm(y, a.n1);
@@ -1215,24 +1243,29 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
       const C* l0 = lds + (r & 1) * LE;
       C* const dst = ring + (r % NR) * SS;
       const bool zero_pl = a.tmode == 0 && !(w0 || (s >= 0 && s < a.n0));
+      // every cell is evaluated before any is stored (the ring stores would
+      // otherwise keep the loads of one cell from serving the next)
+      C res[KA][NF][VZ];
 #pragma unroll
       for (int k = 0; k < KA; ++k) {
-        C res[NF][VZ];
 #pragma unroll
         for (int v = 0; v < VZ; ++v) {
-          Cell<P, R, KA> c{q, l0, k, v, ly[k], lz[k] + v};
+          Cell<P, R, KA, YA> c{q, l0, k, v, ly[k], lz[k] + v};
           C o[NF];
           P::eval(c, o);
 #pragma unroll
-          for (int m = 0; m < NF; ++m) res[m][v] = o[m];
+          for (int m = 0; m < NF; ++m) res[k][m][v] = o[m];
         }
+      }
+#pragma unroll
+      for (int k = 0; k < KA; ++k) {
         const bool zero = a.tmode == 0 && (zero_pl || !keep[k]);
         if (act[k]) {
 #pragma unroll
           for (int m = 0; m < NF; ++m) {
             C* d = dst + ly[k] * RP + m * PR + lz[k];
 #pragma unroll
-            for (int v = 0; v < VZ; ++v) d[v] = zero ? C(0) : res[m][v];
+            for (int v = 0; v < VZ; ++v) d[v] = zero ? C(0) : res[k][m][v];
           }
         }
       }
@@ -1252,10 +1285,20 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
     const C* nb[KB];          // ring cell (row - RY, column - RZ) of my vector, plane slot 0
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
-      const int c = t + k * NBT;
-      const bool act = c < OCELLS;
-      const int cc = act ? c : OCELLS - 1;
-      const int ly = GY + cc / ONTZ, lz = GZ + (cc % ONTZ) * VZ;
+      bool act;
+      int ly, lz;
+      if constexpr (YA) {
+        const int grp = t / ONTZ;
+        act = grp < OTY / KB;
+        ly = GY + (act ? grp : OTY / KB - 1) * KB + k;
+        lz = GZ + (t % ONTZ) * VZ;
+      } else {
+        const int c = t + k * NBT;
+        act = c < OCELLS;
+        const int cc = act ? c : OCELLS - 1;
+        ly = GY + cc / ONTZ;
+        lz = GZ + (cc % ONTZ) * VZ;
+      }
       nb[k] = ring + (ly - P::RY) * RP + (lz - P::RZ);
       const int y = y0 + ly, z = z0 + lz;
       const bool mine = act && y >= 0 && y < a.n1 && z >= 0 && z < a.n2;
@@ -1365,7 +1408,7 @@ ST_HD void tbws_tile(const SweepArgs& a, int tile_z, int tile_y, int xb, int xe)
       for (int k = 0; k < KB; ++k)
 #pragma unroll
         for (int v = 0; v < VZ; ++v) {
-          RingCell<P, R, KB, pmod(r - D, NR)> c{q, nb[k], k, v};
+          RingCell<P, R, KB, pmod(r - D, NR), YA> c{q, nb[k], k, v};
           C o[NF];
           P::eval(c, o);
 #pragma unroll

@@ -132,6 +132,10 @@ class TBlockConfig:
     # threads evaluate level 1 on the region, the other ``nt - na`` level 2
     # on the output tile
     na: int = 0
+    # two-level kernel: a thread's ``kc`` (level 1) or ``kb`` (level 2) cells
+    # lie on adjacent rows of one column instead of ``na`` (``nt - na``)
+    # cells apart, so the in-plane neighbours they share are read once
+    ya: int = 0
 
     @property
     def oty(self) -> int:
@@ -154,7 +158,7 @@ class TBlockConfig:
     @property
     def tag(self) -> str:
         if self.na:
-            return f"tw{self.T}r{self.ty}x{self.tz}v{self.vz}n{self.nt}a{self.na}"
+            return f"tw{self.T}r{self.ty}x{self.tz}v{self.vz}n{self.nt}a{self.na}" + ("y" if self.ya else "")
         return f"tb{self.T}r{self.ty}x{self.tz}v{self.vz}n{self.nt}"
 
     def kernel_config(self) -> KernelConfig:
@@ -370,6 +374,19 @@ def tbws_config(meta: StencilMeta, T: int, storage: str, compute: str, vz: int,
     return None if best is None else best[1]
 
 
+def tbws_adjacent(cfg: "TBlockConfig") -> "TBlockConfig | None":
+    """``cfg`` with each thread's cells on adjacent rows (hint ``tbya``),
+    or None where the row groups do not divide the region or the output
+    tile, or a role has fewer threads than (row groups x vector columns)."""
+    ka, kb = cfg.kc, cfg.kb
+    nbt = cfg.nt - cfg.na
+    if cfg.ty % ka or cfg.oty % kb:
+        return None
+    if cfg.na < (cfg.ty // ka) * (cfg.tz // cfg.vz) or nbt < (cfg.oty // kb) * (cfg.otz // cfg.vz):
+        return None
+    return TBlockConfig(cfg.ty, cfg.tz, cfg.vz, cfg.nt, cfg.gy, cfg.gz, cfg.T, cfg.na, ya=1)
+
+
 def select_tblock(meta: StencilMeta, T: int, storage: str, compute: str, vz: int,
                   hints: "dict | None" = None) -> "TBlockConfig | None":
     """The register/LDS temporal-blocking variant the emitter builds for
@@ -387,6 +404,8 @@ def select_tblock(meta: StencilMeta, T: int, storage: str, compute: str, vz: int
     tb = None
     if ws and tbws_eligible(meta, T):
         tb = tbws_config(meta, T, storage, compute, vz, force=hints.get("tbws_tile"))
+        if tb is not None and int(hints.get("tbya", 0)):
+            tb = tbws_adjacent(tb) or tb       # a geometry without whole row groups keeps the flat order
     if tb is None:
         tb = tblock_config(meta, T, storage, compute, vz, force=hints.get("tblock_tile"))
     return tb
@@ -431,7 +450,7 @@ def emit_traits(meta: StencilMeta, cfg: KernelConfig, storage: str, compute: str
     if tblock is not None:
         lines.append(f"  static constexpr int TT = {tblock.T}, GY = {tblock.gy}, GZ = {tblock.gz}, "
                      f"TBSKIP = {int(tbskip)}, TBDB = {int(tbdb)}, TBPD = {int(tbpd)}, NA = {int(tblock.na)}, "
-                     f"TBRP = {int(tbrp)};")
+                     f"TBRP = {int(tbrp)}, TBYA = {int(tblock.ya)};")
     upd = meta.update
     ne = meta.num_exprs
     if upd is None:

@@ -431,11 +431,17 @@ def _measure_split(stepper, u, min_ms: float, verbose: bool) -> int:
                     stepper.enqueue(a, b, blocks * T, s)
                 g.launch(s)
                 rt.stream_sync(s)
-                e0, e1 = rt.Event(), rt.Event()
-                e0.record(s)
-                g.launch(s)
-                e1.record(s)
-                ms = e0.elapsed_ms(e1)
+                # the median of three replays (the depths of the 25-point
+                # star are within a few per cent of each other, about the
+                # spread of single replays)
+                times = []
+                for _ in range(3):
+                    e0, e1 = rt.Event(), rt.Event()
+                    e0.record(s)
+                    g.launch(s)
+                    e1.record(s)
+                    times.append(e0.elapsed_ms(e1))
+                ms = sorted(times)[1]
                 del g
         except (ValueError, RuntimeError):
             continue

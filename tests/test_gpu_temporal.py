@@ -39,6 +39,9 @@ ZERO_CASES = [
     ("star25", 2, {"tbws": 0}), ("star25", 2, {"tbws_tile": [24, 32, 640, 384]}),
     ("star25", 2, {"tbws_tile": [24, 32, 320, 192]}), ("diffusion7", 2, {"tbws": 1}),
     ("star25-float32", 2, None), ("star25-float32", 2, {"tbws_tile": [24, 64, 576, 384]}),
+    # two cells per thread on adjacent rows (option tbya)
+    ("star25", 2, {"tbws_tile": [24, 32, 320, 192], "tbya": 1}),
+    ("star25-float32", 2, {"tbws_tile": [24, 64, 320, 192], "tbya": 1}),
 ]
 
 
@@ -60,6 +63,8 @@ def test_zero_mode_equals_zero_halo_chain(oracle, which, T, hints):
     assert r.temporal_steps == T and key in v
     if hints and ("tbws_tile" in hints or hints.get("tbws")):
         assert "_tblock_tw" in v[key][0], v[key]
+    if hints and hints.get("tbya"):
+        assert v[key][0].endswith("y"), v[key]
     if hints and hints.get("tbws") == 0:
         assert "_tblock_tb" in v[key][0], v[key]
     print(f"{which} T={T} hints {hints} variant {v[key]}", flush=True)
@@ -84,6 +89,7 @@ def test_zero_mode_equals_zero_halo_chain(oracle, which, T, hints):
     # rows read around every periodic axis), forced geometries
     ("star25", 2, 7, (37, 45, 58), {"tbws_tile": [24, 32, 320, 192]}),
     ("star25", 2, 7, (37, 45, 58), {"tbws_tile": [16, 32, 512, 256]}),
+    ("star25", 2, 7, (37, 45, 58), {"tbws_tile": [24, 32, 320, 192], "tbya": 1}),
     ("diffusion7", 2, 7, (21, 45, 58), {"tbws": 1}),
 ])
 def test_periodic_recompute_march(oracle, which, T, steps, shape, hints):

@@ -86,6 +86,9 @@ DEVICE_CASES = [
     ("star25-float32", 2, (23, 19, 45), 0, {"tbws_tile": [24, 64, 576, 384]}),
     ("star25", 2, (37, 45, 58), "wrap", {"tbws_tile": [24, 32, 320, 192]}),
     ("star25", 2, (37, 45, 58), "wrap", {"tbws_tile": [16, 32, 512, 256]}),
+    ("star25", 2, (23, 19, 45), 0, {"tbws_tile": [24, 32, 320, 192], "tbya": 1}),
+    ("star25-float32", 2, (23, 19, 45), 0, {"tbws_tile": [24, 64, 320, 192], "tbya": 1}),
+    ("star25", 2, (37, 45, 58), "wrap", {"tbws_tile": [24, 32, 320, 192], "tbya": 1}),
     ("diffusion7", 2, (21, 45, 58), "wrap", {"tbws": 1}),
 ]
 DEVICE_CASES = [c if len(c) == 5 else c + (None,) for c in DEVICE_CASES]
@@ -134,18 +137,25 @@ BENCH_CASES = [
     ("diffusion7", "float64", 4, (512, 512, 512), 7),
     ("heat2d", "float64", 4, (1024, 1024), 0),
     ("heat2d", "float64", 10, (1024, 1024), 0),
+    # two-level geometries with two cells per thread on adjacent rows (tbya)
+    ("star25", "float64", 2, (512, 512, 512), 7, {"tbws_tile": [24, 32, 320, 192], "tbya": 1}),
+    ("star25", "float64", 2, (512, 512, 512), 7, {"tbws_tile": [16, 32, 256, 128], "tbya": 1}),
+    ("star25", "float32", 2, (512, 512, 512), 7, {"tbws_tile": [24, 64, 320, 192], "tbya": 1}),
+    ("star25", "float32", 2, (512, 512, 512), 7, {"tbws_tile": [16, 64, 256, 128], "tbya": 1}),
 ]
+BENCH_CASES = [c if len(c) == 6 else c + (None,) for c in BENCH_CASES]
 
 
-@pytest.mark.parametrize("which,dtype,T,n,wrap", BENCH_CASES,
-                         ids=[f"{c[0]}-{c[1]}-T{c[2]}-w{c[4]}" for c in BENCH_CASES])
-def test_bench_launch_in_bounds(which, dtype, T, n, wrap):
+@pytest.mark.parametrize("which,dtype,T,n,wrap,hints", BENCH_CASES,
+                         ids=[f"{c[0]}-{c[1]}-T{c[2]}-w{c[4]}{'-' + str(sorted(c[5].items())) if c[5] else ''}"
+                              for c in BENCH_CASES])
+def test_bench_launch_in_bounds(which, dtype, T, n, wrap, hints):
     fn = getattr(w, which)(dtype)
-    r = compile(fn, temporal_steps=T, temporal_mode="zero")
+    r = compile(fn, temporal_steps=T, temporal_mode="zero", kernel_hints=hints)
     halo = r.halo_widths
     shape = tuple(a + 2 * h for a, h in zip(n, halo))
     kern, key, args = _launch_geometry(r, shape, r.storage)
-    meta, cfg = _cfg_for(fn, r, key, r.storage)
+    meta, cfg = _cfg_for(fn, r, key, r.storage, hints)
     u = np.zeros(shape)
     got, errs = tm.run(fn._fn, len(n), cfg, plan_accesses(meta), tblock_queue_range(meta), u, halo, args.xch,
                        tmode=0, wrap=wrap, edges=True)

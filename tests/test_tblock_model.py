@@ -7,6 +7,8 @@ deep filled halo, periodic wrap) with no access outside its buffer, for
 several chunk lengths and ragged tile counts.  Only geometries that pass here
 run on a device."""
 
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -15,7 +17,8 @@ import tblock_model as tm
 from cutile import workloads as w
 from cutile.frontend.decorator import stencil_from_source
 from cutile.frontend.ir import find_func
-from cutile.lowering.hip_emitter import plan_accesses, radius3, select_tblock, tblock_config, tblock_queue_range
+from cutile.lowering.hip_emitter import (plan_accesses, radius3, select_tblock, tblock_config, tblock_queue_range,
+                                        tbws_adjacent)
 from cutile.lowering.meta import extract_meta
 
 
@@ -72,6 +75,10 @@ CASES = [
     # geometries with two cells per thread
     ("star25-T2-v2-ws24x32", w.star25, 2, "float64", 2, (9, 50, 70), (8, 8, 16), {"tbws_tile": (24, 32, 640, 384)}),
     ("star25-T2-v2-ws2c", w.star25, 2, "float64", 2, (9, 50, 70), (8, 8, 16), {"tbws_tile": (24, 32, 320, 192)}),
+    ("star25-T2-v2-ws2y", w.star25, 2, "float64", 2, (9, 50, 70), (8, 8, 16),
+     {"tbws_tile": (24, 32, 320, 192), "tbya": 1}),
+    ("star25f-T2-v4-ws2y", w.star25, 2, "float32", 4, (9, 50, 136), (8, 8, 32),
+     {"tbws_tile": (24, 64, 320, 192), "tbya": 1}),
     ("diff7-T2-v2-ws", w.diffusion7, 2, "float64", 2, (9, 30, 200), (2, 2, 16), {"tbws": 1}),
     ("diff7-T2-v1-ws", w.diffusion7, 2, "float64", 1, (9, 29, 101), (2, 2, 2), {"tbws": 1}),
     ("diff7-T2-v2-tb", w.diffusion7, 2, "float64", 2, (9, 50, 70), (2, 2, 16), {"tbws": 0}),
@@ -200,3 +207,61 @@ def test_zdpp_hint_selects_the_single_role_kernel():
     assert select_tblock(meta, 2, "float64", "float64", 2).na > 0
     assert select_tblock(meta, 2, "float64", "float64", 2, {"zdpp": 1}).na == 0
     assert select_tblock(meta, 2, "float64", "float64", 2, {"zdpp": 1, "tbws": 1}).na == 0
+
+
+def _ws_cells(cfg):
+    """(level-1, level-2) cells each thread owns in ``st::tbws_tile``:
+    lists of (thread, k, row, vector column) in region coordinates, active
+    cells only, with the kernel's mapping (flat, or adjacent rows when
+    ``cfg.ya``)."""
+    ntz, ontz = cfg.tz // cfg.vz, cfg.otz // cfg.vz
+    ka, kb, na, nbt = cfg.kc, cfg.kb, cfg.na, cfg.nt - cfg.na
+    one, two = [], []
+    for t in range(na):
+        for k in range(ka):
+            if cfg.ya:
+                grp = t // ntz
+                if grp < cfg.ty // ka:
+                    one.append((t, k, grp * ka + k, t % ntz))
+            elif t + k * na < cfg.ty * ntz:
+                c = t + k * na
+                one.append((t, k, c // ntz, c % ntz))
+    for t in range(nbt):
+        for k in range(kb):
+            if cfg.ya:
+                grp = t // ontz
+                if grp < cfg.oty // kb:
+                    two.append((t, k, cfg.gy + grp * kb + k, cfg.gz // cfg.vz + t % ontz))
+            elif t + k * nbt < cfg.oty * ontz:
+                c = t + k * nbt
+                two.append((t, k, cfg.gy + c // ontz, cfg.gz // cfg.vz + c % ontz))
+    return one, two
+
+
+@pytest.mark.parametrize("storage,vz,force", [("float64", 2, (24, 32, 320, 192)), ("float32", 4, (24, 64, 320, 192)),
+                                              ("float64", 2, (16, 32, 256, 128))])
+def test_adjacent_rows_cover_each_cell_once(storage, vz, force):
+    """Option ``tbya`` (each thread's cells on adjacent rows of one column):
+    every region cell has exactly one level-1 owner and every output cell
+    exactly one level-2 owner, as in the flat order; a thread's cells k and
+    k + 1 are one row apart in the same column (what lets the kernel read a
+    neighbour that is its own cell from its queue); and a geometry whose
+    row groups do not divide the tile keeps the flat order."""
+    meta = extract_meta(find_func(w.star25(storage)._ir))
+    flat = select_tblock(meta, 2, storage, storage, vz, {"tbws_tile": force})
+    adj = select_tblock(meta, 2, storage, storage, vz, {"tbws_tile": force, "tbya": 1})
+    assert flat.ya == 0 and adj.ya == 1 and adj.tag == flat.tag + "y"
+    for cfg in (flat, adj):
+        one, two = _ws_cells(cfg)
+        region = sorted((r, c) for _, _, r, c in one)
+        assert region == sorted((r, c) for r in range(cfg.ty) for c in range(cfg.tz // cfg.vz))
+        out = sorted((r, c) for _, _, r, c in two)
+        assert out == sorted((r, c) for r in range(cfg.gy, cfg.gy + cfg.oty)
+                             for c in range(cfg.gz // cfg.vz, (cfg.gz + cfg.otz) // cfg.vz))
+    for cells in _ws_cells(adj):
+        by = {(t, k): (r, c) for t, k, r, c in cells}
+        for (t, k), (r, c) in by.items():
+            if (t, k + 1) in by:
+                assert by[t, k + 1] == (r + 1, c)
+    odd = dataclasses.replace(flat, ty=flat.ty - 1)       # an odd region height: no whole row pairs
+    assert odd.kc == 2 and tbws_adjacent(odd) is None
