@@ -875,6 +875,16 @@ int st_event_sync(void* ev) {
   HIPCHK(hipEventSynchronize((hipEvent_t)ev));
   return ST_OK;
 }
+int st_event_query(void* ev, int* done) {
+  const hipError_t e = hipEventQuery((hipEvent_t)ev);
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return ST_OK;
+  }
+  HIPCHK(e);
+  *done = 1;
+  return ST_OK;
+}
 int st_stream_sync(void* stream) {
   HIPCHK(hipStreamSynchronize((hipStream_t)stream));
   return ST_OK;

@@ -27,7 +27,7 @@ EXPORTS = (
     "st_launch", "st_function_resources", "st_fill_halo", "st_fill_halo_batched", "st_copy", "st_copy_wide",
     "st_diff_norms", "st_residual_norms", "st_residual_norms_static", "st_field_digest", "st_bricks",
     "st_event_create", "st_event_record", "st_event_elapsed_ms", "st_event_destroy", "st_event_sync",
-    "st_stream_sync", "st_device_sync", "st_graph_begin", "st_graph_end", "st_graph_launch",
+    "st_event_query", "st_stream_sync", "st_device_sync", "st_graph_begin", "st_graph_end", "st_graph_launch",
     "st_graph_destroy",
 )
 
@@ -101,6 +101,7 @@ def _declare(lib):
         "st_event_record": ([vp, vp], ctypes.c_int),
         "st_event_elapsed_ms": ([vp, vp, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
         "st_event_sync": ([vp], ctypes.c_int),
+        "st_event_query": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "st_event_destroy": ([vp], ctypes.c_int),
         "st_stream_sync": ([vp], ctypes.c_int),
         "st_device_sync": ([], ctypes.c_int),
@@ -320,6 +321,13 @@ class Event:
 
     def synchronize(self) -> None:
         check(lib().st_event_sync(self.h), "st_event_sync")
+
+    def query(self) -> bool:
+        """True once the work recorded before the event has completed
+        (does not wait)."""
+        done = ctypes.c_int(0)
+        check(lib().st_event_query(self.h, ctypes.byref(done)), "st_event_query")
+        return bool(done.value)
 
     def elapsed_ms(self, end: "Event") -> float:
         ms = ctypes.c_float(0)

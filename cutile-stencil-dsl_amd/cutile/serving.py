@@ -12,24 +12,30 @@ step); this is the request loop of the serving configuration:
 * **resident fields**: an admitted job's buffers are allocated once and
   initialised on the device from ``seed = job id``; the job steps in place
   (ping-pong buffers) until done, then its buffers are released;
-* **packing**: every tick advances every active job by one step -- all jobs
-  of one (stencil, storage) class go through ONE launch of the batched
-  persistent kernel (``cutile.runtime.batched``) over a device work-list of
-  their tiles, with per-size-class tile buckets; periodic halos of a class
-  are refilled by one batched fill launch per axis; Dirichlet halos are
-  zero once and stay so (the sweep writes interiors only);
+* **packing**: a step of a (stencil, storage) class advances every active
+  job of the class by one step -- all of them go through ONE launch of the
+  batched persistent kernel (``cutile.runtime.batched``) over a device
+  work-list of their tiles, with per-size-class tile buckets; periodic
+  halos of a class are refilled by one batched fill launch per axis;
+  Dirichlet halos are zero once and stay so (the sweep writes interiors
+  only);
+* **pacing**: each class has its own stream and steps whenever fewer than
+  ``inflight`` of its steps are queued on the device (completion is polled
+  with events, never waited on while another class can step), so a class
+  of cheap steps is not held to the pace of the slowest class
+  (``pacing="lockstep"``: all classes step together, one tick at a time);
 * **solo temporal stepping**: large jobs (``SOLO``) are stepped alone,
   several steps per launch of the register/LDS temporal kernel (coupled
   fields for Gray-Scott), where that outruns the batched sweep; their last
   steps short of a block run as single sweeps;
-* **continuous batching**: jobs join at the next tick after they are
-  admitted and leave after their last step, so work of jobs that arrived at
-  different times overlaps on the device; the work-lists are rebuilt only
-  when the active set changes (two plans per set: buffer parities flip
-  every tick);
-* latency = completion (the tick of a job's last step has finished on the
-  device) minus arrival; the loop reports p50/p99 latency and aggregate
-  Gcell-updates/s.
+* **continuous batching**: jobs join their class at its next step after
+  they are admitted and leave after their last step, so work of jobs that
+  arrived at different times overlaps on the device; the work-lists are
+  rebuilt only when a class's active set changes (two plans per set:
+  buffer parities flip every step);
+* latency = completion (the class step holding a job's last step has
+  finished on the device, as the host sees it) minus arrival; the loop
+  reports p50/p99 latency and aggregate Gcell-updates/s.
 
 Each job's result depends only on its own fields: the batched kernel
 evaluates every instance with the same arithmetic as the single-instance
@@ -526,19 +532,27 @@ class StencilServer:
     ``hbm_budget``: bytes of resident fields allowed at once (default: 90 %
     of the device's free memory at start).  ``keep_results``: keep each
     job's final fields (device tensors) in ``results`` instead of freeing
-    them.  ``inflight``: ticks queued on the device ahead of the host."""
+    them.  ``inflight``: steps a class has queued on the device ahead of the
+    host.  ``pacing``: ``"class"`` (default) lets every (stencil, storage)
+    class step as soon as its own queue has room, so a class of cheap steps
+    is not held to the pace of the slowest class; ``"lockstep"`` advances
+    all classes together, one step per tick."""
 
-    def __init__(self, hbm_budget: Optional[int] = None, keep_results: bool = False, inflight: int = 3):
+    def __init__(self, hbm_budget: Optional[int] = None, keep_results: bool = False, inflight: int = 3,
+                 pacing: str = "class"):
         import torch
 
         from cutile.runtime import rt
 
+        if pacing not in ("class", "lockstep"):
+            raise ValueError(f"pacing must be 'class' or 'lockstep', got {pacing!r}")
         if hbm_budget is None:
             free, _total = rt.mem_info()
             hbm_budget = int(free * 0.9)
         self.budget = int(hbm_budget)
         self.keep = keep_results
         self.inflight = max(1, inflight)
+        self.pacing = pacing
         self.classes: Dict[tuple, _Class] = {}
         self.results: Dict[int, dict] = {}
         self.reserved = 0
@@ -580,9 +594,11 @@ class StencilServer:
         host = {"admit": 0.0, "rebuild": 0.0, "launch": 0.0, "wait": 0.0, "idle": 0.0}
 
         def retire(ev, done_jobs):
-            tw = time.perf_counter()
-            ev.synchronize()
-            host["wait"] += time.perf_counter() - tw
+            # ev: the event of the step that finished these jobs (None: known complete)
+            if ev is not None:
+                tw = time.perf_counter()
+                ev.synchronize()
+                host["wait"] += time.perf_counter() - tw
             tnow = now()
             for j in done_jobs:
                 j.finished = tnow
@@ -597,7 +613,31 @@ class StencilServer:
                 self.reserved -= j.nbytes
                 finished.append(j)
 
-        while pending or waiting or active or ticks:
+        # class pacing: per class, the steps queued on the device, oldest first:
+        # (event recorded after the step, jobs whose last step it was)
+        queues: Dict[tuple, list] = {}
+
+        def poll() -> bool:
+            """Retire every completed class step without waiting; True if any."""
+            got = False
+            for q in queues.values():
+                while q and q[0][0].query():
+                    retire(None, q.pop(0)[1])
+                    got = True
+            return got
+
+        def wait_any() -> None:
+            """Wait until some queued class step completes and retire it."""
+            tw = time.perf_counter()
+            while any(queues.values()) and not poll():
+                pass
+            host["wait"] += time.perf_counter() - tw
+
+        def outstanding() -> bool:
+            return bool(ticks) or any(queues.values())
+
+        last_log = 0.0
+        while pending or waiting or active or outstanding():
             t = now() if realtime else float("inf")
             while pending and pending[0].arrival <= t:
                 waiting.append(pending.pop(0))
@@ -620,6 +660,9 @@ class StencilServer:
                 if ticks:
                     retire(*ticks.pop(0))
                     continue
+                if any(queues.values()):
+                    wait_any()
+                    continue
                 if pending and realtime:
                     ti = time.perf_counter()
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
@@ -631,37 +674,64 @@ class StencilServer:
                 with torch.cuda.stream(cls.stream):       # work-list uploads on the class's stream
                     cls.rebuild()
             host["rebuild"] += time.perf_counter() - tr
-            # one tick: every class advances its jobs by one step.  A class's
-            # buffers, uploads and launches all live on its own stream, so the
-            # classes never wait for each other: a class's next step starts
-            # as soon as its previous one ends, whatever the other classes
-            # are doing.  The main stream joins them only to mark the tick
-            # for retirement (and the host's in-flight limit).
+            # A class's buffers, uploads and launches all live on its own
+            # stream, so the classes never wait for each other on the device.
             tl = time.perf_counter()
-            busy = [c for c in self.classes.values() if c.jobs]
-            for cls in busy:
-                cls.step(cls.stream.cuda_stream)
-            for cls in busy:
-                self.main.wait_stream(cls.stream)
-            ev = rt.Event()
-            ev.record(self.stream)
-            host["launch"] += time.perf_counter() - tl
-            nticks += 1
-            done_now = [j for j in active if j.done >= j.steps]
-            if done_now:
-                tr = time.perf_counter()
-                active = [j for j in active if j.done < j.steps]
-                for cls in {self._class(j) for j in done_now}:
-                    cls.jobs = [j for j in cls.jobs if j.done < j.steps]
-                    cls.tick = 0
-                    with torch.cuda.stream(cls.stream):
-                        cls.rebuild()
-                host["rebuild"] += time.perf_counter() - tr
-            ticks.append((ev, done_now))
-            while len(ticks) > self.inflight:
-                retire(*ticks.pop(0))
-            if log and nticks % 500 == 0:
-                log(f"serving: t={now():.1f}s ticks={nticks} active={len(active)} waiting={len(waiting)} "
+            if self.pacing == "lockstep":
+                # one tick: every class advances its jobs by one step; the
+                # main stream joins the classes to mark the tick for
+                # retirement (and the host's in-flight limit), so every class
+                # runs at the pace of the slowest
+                busy = [c for c in self.classes.values() if c.jobs]
+                for cls in busy:
+                    cls.step(cls.stream.cuda_stream)
+                for cls in busy:
+                    self.main.wait_stream(cls.stream)
+                ev = rt.Event()
+                ev.record(self.stream)
+                nticks += 1
+                done_now = [j for j in active if j.done >= j.steps]
+                if done_now:
+                    active = [j for j in active if j.done < j.steps]
+                    for cls in {self._class(j) for j in done_now}:
+                        cls.jobs = [j for j in cls.jobs if j.done < j.steps]
+                        cls.tick = 0
+                        with torch.cuda.stream(cls.stream):
+                            cls.rebuild()
+                ticks.append((ev, done_now))
+                host["launch"] += time.perf_counter() - tl
+                while len(ticks) > self.inflight:
+                    retire(*ticks.pop(0))
+            else:
+                # class pacing: a class steps whenever fewer than `inflight`
+                # of its steps are queued, each step marked by an event on
+                # the class's own stream
+                poll()
+                stepped = False
+                for key, cls in self.classes.items():
+                    q = queues.setdefault(key, [])
+                    if not cls.jobs or len(q) >= self.inflight:
+                        continue
+                    st = cls.stream.cuda_stream
+                    cls.step(st)
+                    ev = rt.Event()
+                    ev.record(st)
+                    nticks += 1
+                    done_now = [j for j in cls.jobs if j.done >= j.steps]
+                    if done_now:
+                        cls.jobs = [j for j in cls.jobs if j.done < j.steps]
+                        active = [j for j in active if j.done < j.steps]
+                        cls.tick = 0
+                        with torch.cuda.stream(cls.stream):
+                            cls.rebuild()
+                    q.append((ev, done_now))
+                    stepped = True
+                host["launch"] += time.perf_counter() - tl
+                if not stepped:
+                    wait_any()                    # every busy class has its queue full
+            if log and now() - last_log >= 0.5:
+                last_log = now()
+                log(f"serving: t={now():.1f}s steps={nticks} active={len(active)} waiting={len(waiting)} "
                     f"done={len(finished)} reserved={self.reserved / 2**30:.2f} GiB")
         torch.cuda.synchronize()
         wall = now()
@@ -692,11 +762,13 @@ def main(argv=None) -> int:
     ap.add_argument("--seconds", type=float, default=60.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--budget-gib", type=float, default=None)
-    ap.add_argument("--inflight", type=int, default=3, help="ticks queued on the device ahead of the host")
+    ap.add_argument("--inflight", type=int, default=3, help="steps a class queues on the device ahead of the host")
+    ap.add_argument("--pacing", choices=("class", "lockstep"), default="class",
+                    help="classes step at their own pace (default) or together, one step per tick")
     args = ap.parse_args(argv)
     jobs = generate_jobs(args.rate, args.seconds, args.seed)
     srv = StencilServer(hbm_budget=None if args.budget_gib is None else int(args.budget_gib * 2**30),
-                        inflight=args.inflight)
+                        inflight=args.inflight, pacing=args.pacing)
     srv.run(warmup_jobs(srv), realtime=False)       # build every class's kernels before the clock starts
     stats = srv.run(jobs, realtime=True,
                     log=lambda m: print(f"[serving] {m}", file=sys.stderr, flush=True))

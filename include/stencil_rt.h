@@ -25,6 +25,8 @@
  *                                                    -> st_residual_norms(_static), st_field_digest
  *   (no upstream counterpart: hipGraph capture of step loops)
  *                                                    -> st_graph_*
+ *   (no upstream counterpart: the serving loop's completion polls)
+ *                                                    -> st_event_query
  *
  * Conventions: every function returns 0 on success and a non-zero code
  * (a hipError_t, or a negative ST_E* code) on failure; st_last_error()
@@ -199,6 +201,7 @@ int st_event_record(void* ev, void* stream);
 int st_event_elapsed_ms(void* start, void* stop, float* ms);
 int st_event_destroy(void* ev);
 int st_event_sync(void* ev);         /* wait until the event has completed */
+int st_event_query(void* ev, int* done);   /* *done = 1 if the event has completed, else 0; never waits */
 int st_stream_sync(void* stream);
 int st_device_sync(void);
 

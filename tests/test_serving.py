@@ -54,6 +54,25 @@ def test_each_job_is_the_same_alone_and_in_the_mix():
 
 
 @pytest.mark.gpu
+def test_class_pacing_equals_lockstep():
+    """Classes stepping at their own pace (the default) give every job the
+    same bits as all classes stepping together, one tick at a time."""
+    import torch
+
+    from cutile.serving import StencilServer
+
+    print("serving: class pacing vs lockstep ticks", flush=True)
+    paced = StencilServer(keep_results=True, pacing="class")
+    lock = StencilServer(keep_results=True, pacing="lockstep")
+    a = paced.run(_mix(), realtime=False)
+    b = lock.run(_mix(), realtime=False)
+    assert a["jobs"] == b["jobs"] == len(_mix())
+    for jid, fields in lock.results.items():
+        for f, t in fields.items():
+            assert torch.equal(t, paced.results[jid][f]), (jid, f)
+
+
+@pytest.mark.gpu
 def test_float64_jobs_match_the_oracle(oracle):
     import torch
 
