@@ -78,6 +78,7 @@ class Job:
     fdesc: Optional[list] = field(default=None, repr=False)
     solo: bool = False            # stepped alone, several steps per launch (large jobs)
     cur: int = 0                  # solo jobs: the buffer holding the current state
+    lane_ready: bool = False      # solo jobs (class pacing): the solo stream has waited for its set-up
 
     @property
     def ndim(self) -> int:
@@ -430,8 +431,17 @@ class _Class:
         """Advance every active job of the class: batched jobs by one step
         (one launch for all), solo jobs by T steps (one temporal launch each;
         single sweeps for a job's last steps short of a block)."""
+        self.step_batched(stream)
+        for j in self.jobs:
+            if j.solo:
+                self.step_solo(j, stream)
+
+    def has_batched(self) -> bool:
+        return any(not j.solo for j in self.jobs)
+
+    def step_batched(self, stream) -> None:
+        """Advance the class's batched jobs by one step (one launch for all)."""
         from cutile.runtime import rt
-        from cutile.runtime.kernel import run_prepared
 
         flip = self.tick % 2
         plan = self.plans[flip] if self.plans else None
@@ -444,21 +454,26 @@ class _Class:
         for j in self.jobs:
             if not j.solo:
                 j.done += 1
-                continue
-            sol = self.solo
-            block = j.steps - j.done >= sol.T
-            rec = j.recs[j.cur][block] if j.recs else _UNPREPARED
-            if rec is _UNPREPARED:                # packed once per buffer parity and kind
-                if not j.recs:
-                    j.recs = [[_UNPREPARED, _UNPREPARED], [_UNPREPARED, _UNPREPARED]]
-                ins = [j.bufs[f][j.cur] for f in self.fields]
-                outs = [j.bufs[f][1 - j.cur] for f in self.fields]
-                rec = (sol.block.prepare(ins, outs, sol.halo, temporal=True, mode=0, periodic_axes=sol.wrap)
-                       if block else sol.single.prepare(ins, outs, sol.halo, periodic_axes=sol.wrap))
-                j.recs[j.cur][block] = rec       # None: empty interior, nothing to launch
-            run_prepared(rec, stream)
-            j.done += sol.T if block else 1
-            j.cur = 1 - j.cur
+
+    def step_solo(self, j: Job, stream) -> None:
+        """Advance solo job ``j`` by T steps (one temporal launch), or by one
+        sweep for its last steps short of a block."""
+        from cutile.runtime.kernel import run_prepared
+
+        sol = self.solo
+        block = j.steps - j.done >= sol.T
+        rec = j.recs[j.cur][block] if j.recs else _UNPREPARED
+        if rec is _UNPREPARED:                # packed once per buffer parity and kind
+            if not j.recs:
+                j.recs = [[_UNPREPARED, _UNPREPARED], [_UNPREPARED, _UNPREPARED]]
+            ins = [j.bufs[f][j.cur] for f in self.fields]
+            outs = [j.bufs[f][1 - j.cur] for f in self.fields]
+            rec = (sol.block.prepare(ins, outs, sol.halo, temporal=True, mode=0, periodic_axes=sol.wrap)
+                   if block else sol.single.prepare(ins, outs, sol.halo, periodic_axes=sol.wrap))
+            j.recs[j.cur][block] = rec       # None: empty interior, nothing to launch
+        run_prepared(rec, stream)
+        j.done += sol.T if block else 1
+        j.cur = 1 - j.cur
 
 
 def init_fields(job: Job, cls: _Class, bufs: dict) -> None:
@@ -567,6 +582,7 @@ class StencilServer:
 
             cls = _Class(*key)
             cls.stream = torch.cuda.Stream()
+            cls.solo_stream = torch.cuda.Stream()        # the class's solo jobs (class pacing)
             self.classes[key] = cls
         return self.classes[key]
 
@@ -613,17 +629,20 @@ class StencilServer:
                 self.reserved -= j.nbytes
                 finished.append(j)
 
-        # class pacing: per class, the steps queued on the device, oldest first:
-        # (event recorded after the step, jobs whose last step it was)
+        # class pacing: per class (its batched jobs) and per solo job, the
+        # steps queued on the device, oldest first: (event recorded after the
+        # step, jobs whose last step it was)
         queues: Dict[tuple, list] = {}
 
         def poll() -> bool:
-            """Retire every completed class step without waiting; True if any."""
+            """Retire every completed queued step without waiting; True if any."""
             got = False
-            for q in queues.values():
+            for key, q in list(queues.items()):
                 while q and q[0][0].query():
                     retire(None, q.pop(0)[1])
                     got = True
+                if not q and key[0] == "job":
+                    del queues[key]                # a solo job's queue, re-created while it runs
             return got
 
         def wait_any() -> None:
@@ -703,29 +722,54 @@ class StencilServer:
                 while len(ticks) > self.inflight:
                     retire(*ticks.pop(0))
             else:
-                # class pacing: a class steps whenever fewer than `inflight`
-                # of its steps are queued, each step marked by an event on
-                # the class's own stream
+                # class pacing: a class's batched jobs step (one launch for
+                # all) whenever fewer than `inflight` of their steps are
+                # queued, each step marked by an event on the class's stream;
+                # each solo job likewise, on the class's solo stream, so a
+                # large job's temporal launches do not hold back the small ones
                 poll()
                 stepped = False
                 for key, cls in self.classes.items():
-                    q = queues.setdefault(key, [])
-                    if not cls.jobs or len(q) >= self.inflight:
+                    if not cls.jobs:
                         continue
-                    st = cls.stream.cuda_stream
-                    cls.step(st)
-                    ev = rt.Event()
-                    ev.record(st)
-                    nticks += 1
-                    done_now = [j for j in cls.jobs if j.done >= j.steps]
-                    if done_now:
+                    ended = []
+                    q = queues.setdefault(("class",) + key, [])
+                    if len(q) < self.inflight and cls.has_batched():
+                        st = cls.stream.cuda_stream
+                        cls.step_batched(st)
+                        ev = rt.Event()
+                        ev.record(st)
+                        nticks += 1
+                        done_now = [j for j in cls.jobs if not j.solo and j.done >= j.steps]
+                        q.append((ev, done_now))
+                        ended += done_now
+                        stepped = True
+                    for j in cls.jobs:
+                        if not j.solo or j.done >= j.steps:
+                            continue
+                        qj = queues.setdefault(("job", j.id), [])
+                        if len(qj) >= self.inflight:
+                            continue
+                        if not j.lane_ready:           # its buffers were set up on the class's stream
+                            cls.solo_stream.wait_stream(cls.stream)
+                            j.lane_ready = True
+                        st = cls.solo_stream.cuda_stream
+                        cls.step_solo(j, st)
+                        ev = rt.Event()
+                        ev.record(st)
+                        nticks += 1
+                        last = j.done >= j.steps
+                        qj.append((ev, [j] if last else []))
+                        if last:
+                            ended.append(j)
+                        stepped = True
+                    if ended:
                         cls.jobs = [j for j in cls.jobs if j.done < j.steps]
                         active = [j for j in active if j.done < j.steps]
-                        cls.tick = 0
-                        with torch.cuda.stream(cls.stream):
-                            cls.rebuild()
-                    q.append((ev, done_now))
-                    stepped = True
+                        if any(not j.solo for j in ended):
+                            cls.tick = 0
+                            with torch.cuda.stream(cls.stream):
+                                cls.rebuild()
                 host["launch"] += time.perf_counter() - tl
                 if not stepped:
                     wait_any()                    # every busy class has its queue full
