@@ -12,6 +12,7 @@ the vector or scalar variant from the arrays' alignment, packs the by-value
 from __future__ import annotations
 
 import ctypes
+import functools
 import math
 import os
 import threading
@@ -324,6 +325,7 @@ def sweep_chunk_length(n0: int, tiles: int, default: int, queue_depth: int) -> i
     return max(1, min((c, x) for t, c, x in cands if t <= 1.03 * tmin)[1])
 
 
+@functools.lru_cache(maxsize=4096)
 def tblock_chunk_length(n0: int, tiles: int, default: int, depth: int, slots: int, cus: int = 256) -> int:
     """Planes per workgroup of a temporal-blocking launch.
 

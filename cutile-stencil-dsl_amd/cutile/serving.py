@@ -238,6 +238,8 @@ class _Class:
         self.plans = None
         self.fills = None
         self.tick = 0
+        self._nb = 0                  # batched jobs in the plans (rebuild)
+        self._end_tick = 0            # class step at which the first of them ends
         self.stream = None            # the class's own stream in the server
         self.solo = None
         if self.solo_min:
@@ -389,6 +391,10 @@ class _Class:
 
         self.plans, self.fills = [], []
         batched = [j for j in self.jobs if not j.solo]
+        # batched jobs advance one step per class step from here (tick 0):
+        # the first of them ends at this tick
+        self._end_tick = min((j.steps - j.done for j in batched), default=0)
+        self._nb = len(batched)
         for flip in (0, 1):
             if not batched:
                 self.plans.append(None)
@@ -437,7 +443,12 @@ class _Class:
                 self.step_solo(j, stream)
 
     def has_batched(self) -> bool:
-        return any(not j.solo for j in self.jobs)
+        """Batched jobs in the active set (as of the last ``rebuild``)."""
+        return self._nb > 0
+
+    def batched_ending(self) -> bool:
+        """Some batched job has taken its last step (since the last ``rebuild``)."""
+        return self._nb > 0 and self.tick >= self._end_tick
 
     def step_batched(self, stream) -> None:
         """Advance the class's batched jobs by one step (one launch for all)."""
@@ -609,6 +620,11 @@ class StencilServer:
         nticks = 0
         host = {"admit": 0.0, "rebuild": 0.0, "launch": 0.0, "wait": 0.0, "idle": 0.0}
 
+        spare: list = []                       # events of retired class-paced steps, recorded again
+
+        def new_event():
+            return spare.pop() if spare else rt.Event()
+
         def retire(ev, done_jobs):
             # ev: the event of the step that finished these jobs (None: known complete)
             if ev is not None:
@@ -639,7 +655,9 @@ class StencilServer:
             got = False
             for key, q in list(queues.items()):
                 while q and q[0][0].query():
-                    retire(None, q.pop(0)[1])
+                    ev, done_jobs = q.pop(0)
+                    retire(None, done_jobs)
+                    spare.append(ev)
                     got = True
                 if not q and key[0] == "job":
                     del queues[key]                # a solo job's queue, re-created while it runs
@@ -737,10 +755,11 @@ class StencilServer:
                     if len(q) < self.inflight and cls.has_batched():
                         st = cls.stream.cuda_stream
                         cls.step_batched(st)
-                        ev = rt.Event()
+                        ev = new_event()
                         ev.record(st)
                         nticks += 1
-                        done_now = [j for j in cls.jobs if not j.solo and j.done >= j.steps]
+                        done_now = ([j for j in cls.jobs if not j.solo and j.done >= j.steps]
+                                    if cls.batched_ending() else [])
                         q.append((ev, done_now))
                         ended += done_now
                         stepped = True
@@ -755,7 +774,7 @@ class StencilServer:
                             j.lane_ready = True
                         st = cls.solo_stream.cuda_stream
                         cls.step_solo(j, st)
-                        ev = rt.Event()
+                        ev = new_event()
                         ev.record(st)
                         nticks += 1
                         last = j.done >= j.steps
