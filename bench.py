@@ -628,6 +628,90 @@ def cpu_baseline(n=256, steps=32):
     }
 
 
+def _cpu_c4_instance(args):
+    """One wave instance of the batched configuration, float64, on the NumPy
+    oracle: (cell-updates, seconds of the step loop)."""
+    side, cx, cy, steps = args
+    import numpy as np
+
+    import oracle
+    from cutile.workloads import dirichlet_spec, wave2d
+
+    x = np.arange(side, dtype=np.float64)
+    sig = 0.05 * side
+    u = np.zeros((side + 2, side + 2))
+    u[1:-1, 1:-1] = np.exp(-(((x - cx * side) ** 2)[:, None] + ((x - cy * side) ** 2)[None, :]) / (2 * sig * sig))
+    t0 = time.perf_counter()
+    oracle.leapfrog(u, u.copy(), wave2d("float64")._fn, 2, (1, 1), steps, dirichlet_spec(2))
+    return side * side * steps, time.perf_counter() - t0
+
+
+def _cpu_c5_job(args):
+    """One job of the serving stream, float64, on the NumPy oracle (its own
+    seeded initial field): (cell-updates, seconds of the step loop)."""
+    kind, side, steps, seed = args
+    from cutile import workloads as w
+
+    import oracle
+
+    if kind == "heat2d":
+        u = w.heat2d_init(side, (1, 1), seed=seed)
+        t0 = time.perf_counter()
+        oracle.time_march(u, w.heat2d()._fn, 2, (1, 1), steps, w.dirichlet_spec(2))
+        cells = side * side
+    elif kind == "diffusion3d":
+        u = w.sinusoid_init(side, (1, 1, 1), seed=seed)
+        t0 = time.perf_counter()
+        oracle.time_march(u, w.diffusion7(coef=0.1)._fn, 3, (1, 1, 1), steps, w.dirichlet_spec(3))
+        cells = side ** 3
+    else:
+        u, v = w.gray_scott_init(side, (1, 1), seed=seed)
+        t0 = time.perf_counter()
+        oracle.time_march_multi({"u": u, "v": v}, [f._fn for f in w.gray_scott()], ["u", "v"], 2, (1, 1), steps,
+                                w.periodic_spec(2))
+        cells = side * side
+    return cells * steps, time.perf_counter() - t0
+
+
+def cpu_pools(c4_steps=8, c5_jobs=96, c5_steps=3):
+    """The NumPy oracle over independent work on a process pool (all cores of
+    the affinity set, at most 16): every instance of the batched wave
+    configuration for ``c4_steps`` steps, and the first ``c5_jobs`` jobs of
+    the serving stream for ``c5_steps`` steps each.  Run before the process
+    touches the GPU (the workers are forked).  Per-core rates come from the
+    summed task times."""
+    import multiprocessing as mp
+
+    import numpy as np
+
+    sys.path.insert(0, ROOT)
+    from cutile.serving import generate_jobs
+    from cutile.workloads import wave_sides
+
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    sides = wave_sides(512, seed=0)
+    centres = np.random.default_rng(1).uniform(0, 1, size=(512, 2))
+    c4 = [(int(sd), float(c[0]), float(c[1]), c4_steps) for sd, c in zip(sides, centres)]
+    c4.sort(key=lambda t: -t[0])                      # largest first: an even finish
+    c5 = [(j.kind, j.side, c5_steps, j.id) for j in generate_jobs(200.0, 60.0, 0)[:c5_jobs]]
+    c5.sort(key=lambda t: -(t[1] ** (3 if t[0] == "diffusion3d" else 2)))
+    out = {}
+    with mp.get_context("fork").Pool(cores) as pool:
+        for name, fn, tasks, what in (("c4", _cpu_c4_instance, c4, f"512 wave instances x {c4_steps} steps, float64"),
+                                      ("c5", _cpu_c5_job, c5, f"first {c5_jobs} jobs of the 60-s stream x "
+                                                              f"{c5_steps} steps, float64")):
+            t0 = time.perf_counter()
+            res = pool.map(fn, tasks, chunksize=1)
+            wall = time.perf_counter() - t0
+            upd = sum(r[0] for r in res)
+            busy = sum(r[1] for r in res)
+            out[name] = {"value": upd / wall / 1e9, "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
+                         "per_core": upd / busy / 1e9, "seconds": wall, "sample": what}
+            log(f"cpu pool {name}: {out[name]['value']:.3f} Gcell-updates/s on {cores} cores "
+                f"({out[name]['per_core']:.3f} per core), {wall:.1f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -649,6 +733,9 @@ def main():
         print("bench.py drives one GPU: this build runs every workload on a single MI355X "
               "(no multi-GPU path); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    # the CPU pools fork their workers: before anything touches the GPU
+    pools = None if args.no_cpu or args.no_extra or args.core or args.only else cpu_pools()
 
     import numpy as np
     import torch
@@ -704,6 +791,8 @@ def main():
                 log(f"{key} failed: {e}")
             torch.cuda.empty_cache()
 
+    if pools is not None:
+        extra["cpu_pools"] = pools
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (NumPy oracle, 256^3 sample) ...")
