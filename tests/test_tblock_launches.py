@@ -142,6 +142,11 @@ BENCH_CASES = [
     ("star25", "float64", 2, (512, 512, 512), 7, {"tbws_tile": [16, 32, 256, 128], "tbya": 1}),
     ("star25", "float32", 2, (512, 512, 512), 7, {"tbws_tile": [24, 64, 320, 192], "tbya": 1}),
     ("star25", "float32", 2, (512, 512, 512), 7, {"tbws_tile": [16, 64, 256, 128], "tbya": 1}),
+    ("star25", "float64", 2, (512, 512, 512), 7, {"tbws_tile": [32, 32, 448, 256]}),
+    ("star25", "float64", 2, (512, 512, 512), 7, {"tbws_tile": [32, 32, 448, 256], "tbya": 1}),
+    ("star25", "float64", 2, (512, 512, 512), 7, {"tbws_tile": [32, 32, 448, 256], "tbya": 1, "tbpd": 1,
+                                                  "tbrp": 2}),
+    ("star25", "float64", 2, (512, 512, 512), 7, {"tbws_tile": [32, 32, 448, 256], "tbya": 1, "tbrp": 3}),
 ]
 BENCH_CASES = [c if len(c) == 6 else c + (None,) for c in BENCH_CASES]
 
