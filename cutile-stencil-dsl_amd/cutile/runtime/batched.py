@@ -75,6 +75,9 @@ class BatchPlan:
     cells: int
     instances: int
     keep: list = field(default_factory=list)   # tensors referenced by the table
+    # (device, function, packed kernel arguments) of the last launch: a
+    # plan replayed every step packs its arguments once
+    packed: Optional[tuple] = field(default=None, repr=False)
 
 
 class BatchedStencil:
@@ -277,10 +280,14 @@ class BatchedStencil:
 
         if plan.nitems == 0:
             return
-        args = _BatchArgs(plan.table.data_ptr(), plan.first.data_ptr(), plan.bucket.data_ptr(), plan.instances,
-                     plan.nitems)
-        rt.launch(self._function(), (plan.grid, 1, 1), (self.threads, 1, 1), args,
-                  stream if stream is not None else default_stream())
+        fn = self._function()
+        if plan.packed is None or plan.packed[0] != fn:
+            args = _BatchArgs(plan.table.data_ptr(), plan.first.data_ptr(), plan.bucket.data_ptr(), plan.instances,
+                              plan.nitems)
+            plan.packed = (fn, args, ctypes.addressof(args), ctypes.sizeof(args))
+        _fn, _args, addr, size = plan.packed
+        rt.check(rt.lib().st_launch(fn, plan.grid, 1, 1, self.threads, 1, 1, 0,
+                                    stream if stream is not None else default_stream(), addr, size), "st_launch")
 
 
 class LeapfrogBatch:
