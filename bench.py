@@ -234,6 +234,7 @@ def temporal_depths(interior_dev, n, steps=50):
 
     out = {}
     for sname, mk in (("star25", star25), ("diffusion7", diffusion7)):
+        ref64 = None                  # the float64 state after `steps` steps (bit-equal to the oracle's)
         for dt_name in ("float64", "float32"):
             first = None
             for T in (1, 2, 4):
@@ -248,7 +249,12 @@ def temporal_depths(interior_dev, n, steps=50):
                 state = u[inner].clone()
                 if first is None:
                     first = state
+                    if dt_name == "float64":
+                        ref64 = state
                 same = bool(torch.equal(state, first))
+                err = None
+                if dt_name != "float64" and ref64 is not None:
+                    err = float((state.to(torch.float64) - ref64).abs().max() / ref64.abs().max())
                 g = _full_graph(st, u, steps, rt)
                 ms = _replay_ms(g, rt)
                 torch.cuda.synchronize()
@@ -257,10 +263,12 @@ def temporal_depths(interior_dev, n, steps=50):
                 if st.block_reps > 1:
                     kind += f", {st.block_reps} launches of depth {st.T // st.block_reps} per block"
                 out[key] = {"T": T, "dtype": dt_name, "steps": steps, "halo": list(st.halo), "kernel": kind,
+                            **({"rel_err_vs_float64": err} if err is not None else {}),
                             "ms_per_step": ms / steps, "gcell_updates_per_s": n ** 3 * steps / (ms * 1e-3) / 1e9,
                             "alg_bytes_per_update": 2 * u.element_size() / T, "state_equals_T1": same}
                 log(f"{key}: {ms / steps:.4f} ms/step, {out[key]['gcell_updates_per_s']:.1f} Gcell-updates/s "
-                    f"({kind}; state after {steps} steps equal to T=1: {same})")
+                    f"({kind}; state after {steps} steps equal to T=1: {same}"
+                    + (f"; rel max err vs float64 {err:.2e})" if err is not None else ")"))
                 del u, st, g, state
                 torch.cuda.empty_cache()
             # the depth split the autotuner picks for a block of 4 steps
