@@ -649,16 +649,24 @@ class StencilServer:
         # steps queued on the device, oldest first: (event recorded after the
         # step, jobs whose last step it was)
         queues: Dict[tuple, list] = {}
+        finishing: set = set()                 # queues holding a step that ends some job
 
-        def poll() -> bool:
-            """Retire every completed queued step without waiting; True if any."""
+        def poll(every: bool = False) -> bool:
+            """Retire the completed steps of every full queue and of every
+            queue that ends a job (``every``: of all queues), without
+            waiting; True if any.  (A queue with room and no ending job needs
+            no retirement to go on.)"""
             got = False
             for key, q in list(queues.items()):
+                if not every and len(q) < self.inflight and key not in finishing:
+                    continue
                 while q and q[0][0].query():
                     ev, done_jobs = q.pop(0)
                     retire(None, done_jobs)
                     spare.append(ev)
                     got = True
+                if key in finishing and not any(d for _, d in q):
+                    finishing.discard(key)
                 if not q and key[0] == "job":
                     del queues[key]                # a solo job's queue, re-created while it runs
             return got
@@ -666,7 +674,7 @@ class StencilServer:
         def wait_any() -> None:
             """Wait until some queued class step completes and retire it."""
             tw = time.perf_counter()
-            while any(queues.values()) and not poll():
+            while any(queues.values()) and not poll(every=True):
                 pass
             host["wait"] += time.perf_counter() - tw
 
@@ -761,6 +769,8 @@ class StencilServer:
                         done_now = ([j for j in cls.jobs if not j.solo and j.done >= j.steps]
                                     if cls.batched_ending() else [])
                         q.append((ev, done_now))
+                        if done_now:
+                            finishing.add(("class",) + key)
                         ended += done_now
                         stepped = True
                     for j in cls.jobs:
@@ -781,6 +791,7 @@ class StencilServer:
                         qj.append((ev, [j] if last else []))
                         if last:
                             ended.append(j)
+                            finishing.add(("job", j.id))
                         stepped = True
                     if ended:
                         cls.jobs = [j for j in cls.jobs if j.done < j.steps]
