@@ -43,12 +43,12 @@ def test_workload_kernels_build_without_scratch():
     from cutile import compile, compile_fused
     from cutile import workloads as w
     from cutile.runtime import regcheck, rt
-    from cutile.runtime.kernel import compile_options
+    from cutile.runtime.kernel import code_object
 
     results = [compile(w.heat2d()), compile(w.diffusion7()), compile(w.star25()), compile(w.star25("float32")),
                compile(w.wave2d()), compile_fused(list(w.gray_scott()))]
     for r in results:
-        res = regcheck.code_object_resources(rt.compile_only(r.hip_source, compile_options("gfx950")))
+        res = regcheck.code_object_resources(code_object(r.hip_source, "gfx950"))
         assert res, r.name
         for name, v in res.items():
             assert v.stack_bytes == 0, f"{name} spills"

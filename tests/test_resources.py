@@ -7,7 +7,7 @@ import pytest
 from cutile import compile, compile_fused
 from cutile import workloads as w
 from cutile.runtime import regcheck, rt
-from cutile.runtime.kernel import compile_options
+from cutile.runtime.kernel import code_object
 
 pytestmark = pytest.mark.skipif(not rt.available() or regcheck.readelf_path() is None,
                                 reason="needs libstencil_rt.so (hiprtc) and llvm-readelf")
@@ -56,7 +56,7 @@ def _cases():
 @pytest.mark.parametrize("name,make", _cases(), ids=[c[0] for c in _cases()])
 def test_default_variants_fit_registers(name, make):
     r = make()
-    res = regcheck.code_object_resources(rt.compile_only(r.hip_source, compile_options("gfx950")))
+    res = regcheck.code_object_resources(code_object(r.hip_source, "gfx950"))
     kernels = {k: v for k, v in res.items() if "_sweep_" in k or "_temporal_" in k or "_tblock_" in k}
     assert kernels, res
     for k, v in kernels.items():
@@ -78,7 +78,7 @@ def test_random_stencils_build_within_lds(seed):
     nd, nf, rad = _RANDOM_SHAPES[seed % len(_RANDOM_SHAPES)]
     fn, _, _ = random_stencil(seed, nd, nf, rad, 12)
     r = compile(fn, temporal_blocking=False)
-    res = regcheck.code_object_resources(rt.compile_only(r.hip_source, compile_options("gfx950")))
+    res = regcheck.code_object_resources(code_object(r.hip_source, "gfx950"))
     for k, v in res.items():
         assert v.lds_bytes <= 160 * 1024, (k, v)
 
@@ -101,7 +101,7 @@ def test_sweep_lds_matches_the_host_estimate(fn, ty, tz, vz, nt):
     storage, compute = storage_compute(meta.dtype, None)
     cfg = KernelConfig(ty=ty, tz=tz, vz=vz, nt=nt)
     src = emit_hip_source(meta, [cfg], storage, compute)
-    res = regcheck.code_object_resources(rt.compile_only(src, compile_options("gfx950")))
+    res = regcheck.code_object_resources(code_object(src, "gfx950"))
     (v,) = res.values()
     assert v.lds_bytes == lds_bytes(meta, cfg, compute)
 

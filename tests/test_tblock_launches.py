@@ -30,7 +30,7 @@ def _mi355x(monkeypatch):
 
     def resources(self, key="vec"):
         # what the device would report, from the code object's notes
-        res = regcheck.code_object_resources(rt.compile_only(self.source, K.compile_options("gfx950")))
+        res = regcheck.code_object_resources(K.code_object(self.source, "gfx950"))
         v = res[self.variants[key][0]]
         return rt.Resources(num_regs=v.registers, local_bytes=v.stack_bytes, shared_bytes=v.lds_bytes)
 
