@@ -42,7 +42,7 @@ def test_cpu_calls_without_a_device():
 def test_workload_kernels_build_without_scratch():
     from cutile import compile, compile_fused
     from cutile import workloads as w
-    from cutile.runtime import regcheck, rt
+    from cutile.runtime import regcheck
     from cutile.runtime.kernel import code_object
 
     results = [compile(w.heat2d()), compile(w.diffusion7()), compile(w.star25()), compile(w.star25("float32")),
