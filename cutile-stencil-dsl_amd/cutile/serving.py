@@ -824,7 +824,8 @@ class StencilServer:
 
 
 def main(argv=None) -> int:
-    """``python -m cutile.serving [--rate R] [--seconds S] [--seed N] [--inflight K]``: the
+    """``python -m cutile.serving [--rate R] [--seconds S] [--seed N] [--inflight K]
+    [--pacing class|lockstep]``: the
     serving configuration (Poisson arrivals, mixed jobs) on the current
     device; prints one JSON line of latency and throughput figures."""
     import argparse
